@@ -1,0 +1,141 @@
+"""Throughput of the PIC hot path (push + Esirkepov + Yee) on one MI355X.
+
+Workload (default): periodic 256^3 grid, warm isotropic electron plasma at
+vth = 0.05c, 32 particles per cell (536,870,912 particles, SoA FP32), first
+order (CIC) Esirkepov current, relativistic Boris, Yee B/2 -> E -> B/2,
+omega_p dt = 0.1, cfl 0.5.  A step is one time_loop_electrodynamic call
+(evolve.py:16-103) executed by the allocation-free runner, with a stable cell
+re-sort every --sort-every steps (counted in the timed region).
+
+Prints ONE JSON line.  value = particle-steps/s over the timed steps.
+roofline: the fused particle kernel (gather + Boris + Esirkepov + position
+update + boundaries), algorithmic bytes 50 per particle (read x,y,z,vx,vy,vz
+and the active byte, write them back) + 36 per cell (E, B gather reads and the
+J write), divided by its mean launch duration from HIP events recorded on the
+stream the kernel runs on.
+cpu_baseline: the FP64 NumPy oracle (oracle/, one core) on a bounded sample.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s (push+Esirkepov+Yee) 256³·32ppc; HBM GB/s; adjoint/fwd cost"
+
+
+def cpu_baseline(n=32, ppc=16):
+    """FP64 NumPy oracle, one electrodynamic Esirkepov step of an n^3 x ppc warm plasma."""
+    import numpy as np
+    from oracle import pic_oracle as O
+    p = O.Params(N=(n, n, n), wind=(float(n),) * 3, g=2, dt=0.5 / 3, current_deposition="esirkepov")
+    rng = np.random.default_rng(20261202)
+    P = n ** 3 * ppc
+    x = np.stack([rng.uniform(-n / 2, n / 2, (1, P)) for _ in range(3)], -1).astype(np.float32).astype(np.float64)
+    u = (rng.standard_normal((1, P, 3)) * 0.05).astype(np.float32).astype(np.float64)
+    sp = O.Species(np.array([-1.0]), np.array([1.0]), np.array([0.36 / 32]), np.ones((1, 3), bool),
+                   np.ones((1, 3), bool))
+    z = [np.zeros(p.shape) for _ in range(3)]
+    st = O.State(x, u, np.ones((1, P), bool), z, [c.copy() for c in z])
+    t0 = time.perf_counter()
+    O.time_loop_electrodynamic(st, sp, p)
+    dt = time.perf_counter() - t0
+    return {"value": P / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 NumPy oracle, one electrodynamic Esirkepov step, {n}^3 x {ppc} ppc "
+                      f"({P} particles), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--ppc", type=int, default=32)
+    ap.add_argument("--sort-every", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU (domain-decomposed) path is "
+                                   "design-only, so --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+    from pypic3d_amd import workloads
+    from pypic3d_amd.runner import ElectrodynamicRunner
+
+    torch.cuda.set_device(0)
+    static, dyn, species = workloads.config_warm_3d(n=args.n, ppc=args.ppc, vth=0.05)
+    parts, cfg = workloads.plasma(static, dyn, species, seed=20261202)
+    fields = workloads.empty_fields(static, dyn)
+    runner = ElectrodynamicRunner(parts, cfg, fields, static, dyn, sort_every=args.sort_every)
+    del parts
+    torch.cuda.synchronize()
+    n_particles = runner.S * runner.P
+    n_cells = args.n ** 3
+    print(f"[bench] {n_particles} particles on {args.n}^3, warmup {args.warmup}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        runner.step()
+    torch.cuda.synchronize()
+
+    runner.enable_timing(True)
+    sorts0 = runner.sorts_done
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        runner.step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    phases = runner.phase_ms()
+    sorts = runner.sorts_done - sorts0
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s; phases {phases}", file=sys.stderr, flush=True)
+
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = n_particles * args.steps / elapsed
+    kern_ms = phases.get("particles")
+    kern_bytes = 50 * n_particles + 36 * n_cells
+    achieved = kern_bytes / (kern_ms * 1e-3) / 1e9 if kern_ms else None
+    step_bytes = 50 * n_particles + 156 * n_cells
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"warm isotropic electron plasma {args.n}^3 x {args.ppc} ppc, vth 0.05c, periodic, "
+                        "CIC Esirkepov, relativistic Boris, Yee",
+            "particles": n_particles, "cells": n_cells, "sort_every": args.sort_every,
+            "resorted_particles_per_step": n_particles * sorts / args.steps,
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": "fused_push_esirkepov_kernel",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": None, "kernel_ms": kern_ms, "algorithmic_bytes": kern_bytes,
+        },
+        "step_hbm_gbs": step_bytes * args.steps / elapsed / 1e9,
+        "phase_ms": phases,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (NumPy oracle sample)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

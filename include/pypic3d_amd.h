@@ -1,0 +1,173 @@
+/*
+ * pypic3d_amd.h -- C ABI of the MI355X (gfx950) PIC hot-path library
+ * libpypic3d_hip.so.
+ *
+ * Every entry point takes plain device pointers, POD descriptors and a
+ * hipStream_t passed as void*.  No torch types cross this boundary.  Return
+ * codes: 0 success, <0 invalid argument (the Python layer raises ValueError),
+ * >0 a hipError_t (RuntimeError).  All work is stream-ordered; no entry point
+ * synchronises the host except pic_poisson_cg (which must read its residual)
+ * and none allocates device memory: scratch comes in through `workspace`
+ * arguments sized by the matching *_workspace_bytes query.
+ *
+ * Layout (one tile covering the domain, i.e. the reference's single-device
+ * layout with the leading (1,1,1) tile axes dropped):
+ *   field component   float32[(Nx+2g)][(Ny+2g)][(Nz+2g)], z fastest
+ *   particle planes   float32[S][P] per component (x, y, z, vx, vy, vz),
+ *                     uint8 active[S][P]; plane index s*P + p
+ * Reference citations are PyPIC3D/<file>:<line>.
+ */
+#ifndef PYPIC3D_AMD_H
+#define PYPIC3D_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIC_ABI_VERSION 1
+#define PIC_MAX_SPECIES 16
+
+/* Scalar run description: StaticParameters + DynamicParameters fields the hot
+ * path reads (parameters.py:14-53) plus the tile-local grid lines
+ * (grids.py:80-111) reduced to origin/spacing, computed on the host in FP64
+ * exactly as the reference builds them. */
+typedef struct PicDesc {
+    int32_t n[3];          /* physical cells Nx, Ny, Nz                       */
+    int32_t g;             /* guard cells                                     */
+    int32_t dims[3];       /* n + 2g                                          */
+    int32_t shape_factor;  /* 1 = CIC, 2 = TSC                                */
+    int32_t relativistic;  /* relativistic Boris (boris.py:60-124)            */
+    int32_t bc[3];         /* field BC: 0 periodic, 1 conducting              */
+    int32_t pbc[3];        /* particle BC: 0 periodic, 1 reflect, 2 absorb    */
+    int32_t pad0;
+    double d[3];           /* dx, dy, dz                                      */
+    double wind[3];        /* x_wind, y_wind, z_wind                          */
+    double org_c[3];       /* tiled "center" line: first coordinate           */
+    double sp_c[3];        /*                      line[1] - line[0]          */
+    double org_v[3];       /* tiled "vertex" line                             */
+    double sp_v[3];
+    double dt, C, eps, mu, alpha;
+} PicDesc;
+
+/* SpeciesConfig (particles/particle_class.py:5-14). */
+typedef struct PicSpecies {
+    int32_t count;
+    int32_t slots;                    /* P: slots per species               */
+    double charge[PIC_MAX_SPECIES];
+    double mass[PIC_MAX_SPECIES];
+    double weight[PIC_MAX_SPECIES];
+    uint8_t update_x[PIC_MAX_SPECIES][3];
+    uint8_t update_u[PIC_MAX_SPECIES][3];
+} PicSpecies;
+
+/* SoA particle planes (TiledParticles x, u, active; particle_class.py:17-31). */
+typedef struct PicParticles {
+    float* x[3];
+    float* u[3];
+    uint8_t* active;
+} PicParticles;
+
+int pic_abi_version(void);
+
+/* ---- field utilities -------------------------------------------------- */
+/* Fill n floats with value (graph-capture safe replacement for memset). */
+int pic_fill_f32(float* ptr, int64_t n, float value, void* stream);
+/* update_tiled_ghost_cells for one tile (ghost_cells.py:142-215, 615-650).
+ * bc_type 0 uses desc->bc, 1 uses desc->pbc (ghost_cells.py:33-39). */
+int pic_halo_refresh(const PicDesc* desc, float* field, int bc_type, void* stream);
+/* fold_tiled_ghost_cells (ghost_cells.py:218-316, 703-736). */
+int pic_halo_fold(const PicDesc* desc, float* field, int bc_type, void* stream);
+/* apply_tiled_zero_boundary / apply_tiled_constant_boundary on one axis
+ * (ghost_cells.py:344-386, 653-700); both refresh afterwards. */
+int pic_zero_boundary(const PicDesc* desc, float* field, int axis, void* stream);
+int pic_constant_boundary(const PicDesc* desc, float* field, int axis, void* stream);
+/* digital_filter / bilinear_filter (filters.py:234-291): out interior <- 3x3x3
+ * VALID correlation of in; out ghosts <- in ghosts.  in != out. */
+int pic_filter_digital(const PicDesc* desc, const float* in, float* out, double alpha, void* stream);
+int pic_filter_bilinear(const PicDesc* desc, const float* in, float* out, void* stream);
+
+/* ---- Yee solver (solvers/first_order_yee.py) --------------------------- */
+/* B_out interior = B_in - (dt/2) curl E with E neighbours read as refreshed
+ * ghosts (first_order_yee.py:96-142); then B_out ghosts refreshed.  The
+ * do_filter branch (:145-159) is pic_filter_digital + pic_halo_refresh. */
+int pic_yee_update_b(const PicDesc* desc, const float* const E[3], const float* const B_in[3],
+                     float* const B_out[3], void* stream);
+/* E_out interior = E_in + dt (C^2 curl B - J/eps) (first_order_yee.py:12-74);
+ * then E_out ghosts refreshed.  Filter and conducting walls are separate calls. */
+int pic_yee_update_e(const PicDesc* desc, const float* const E_in[3], const float* const B[3],
+                     const float* const J[3], float* const E_out[3], void* stream);
+
+/* ---- particles --------------------------------------------------------- */
+/* particle_push (pusher/particle_push.py:13-175): gather E+E_ext, B+B_ext on
+ * the Yee-staggered grids, Boris or relativistic Boris; writes u_out (may
+ * alias u).  E_ext/B_ext may be NULL (zero external field). */
+int pic_push(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+             const float* const E[3], const float* const B[3],
+             const float* const E_ext[3], const float* const B_ext[3],
+             float* const u_out[3], void* stream);
+/* update_tiled_particle_positions (particle_tile_communication.py:82-99) with
+ * step dt*dt_scale; x_out may alias in->x. */
+int pic_advance_positions(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                          double dt_scale, float* const x_out[3], void* stream);
+/* refresh_tiled_particle_tiles on one device (particle_tile_communication.py:
+ * 41-59, 297-375): wrap/reflect/absorb per axis, zero x,u of inactive slots.
+ * In place on io. */
+int pic_particle_boundaries(const PicDesc* desc, const PicSpecies* sp, PicParticles* io, void* stream);
+/* Raw (unfolded) deposits into zero-initialised ghosted arrays:
+ * Esirkepov_current deposit (deposition/Esirkepov.py:105-331),
+ * J_from_rhov deposit (deposition/J_from_rhov.py:88-200),
+ * compute_rho deposit (deposition/rho.py:72-153). */
+int pic_deposit_esirkepov(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                          float* const J[3], void* stream);
+int pic_deposit_direct(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                       float* const J[3], void* stream);
+int pic_deposit_rho(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                    float* rho, void* stream);
+/* Fused particle phase of time_loop_electrodynamic's Esirkepov branch
+ * (evolve.py:36-43, 70-79): gather + push, raw Esirkepov deposit into J (zeroed
+ * by the caller), x += v dt, particle boundaries.  Writes out (may alias in). */
+int pic_fused_push_esirkepov(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                             const float* const E[3], const float* const B[3],
+                             const float* const E_ext[3], const float* const B_ext[3],
+                             PicParticles* out, float* const J[3], void* stream);
+
+/* ---- cell sort (particle_initialization.py:89-102, 140-160) ------------- */
+/* key = s*(N+1) + (ix*Ny+iy)*Nz+iz with ix = clip(floor((x+W/2)/dx),0,Nx-1);
+ * inactive slots get key s*(N+1) + N. */
+int pic_cell_keys(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                  int32_t* keys, void* stream);
+int64_t pic_sort_workspace_bytes(int64_t n);
+/* Stable radix sort of (key, slot) pairs: perm = stable argsort(keys);
+ * keys_sorted receives the sorted keys. */
+int pic_sort_permutation(const int32_t* keys, int32_t* keys_sorted, int32_t* perm, int64_t n,
+                         int32_t key_bits, void* workspace, int64_t workspace_bytes, void* stream);
+/* out planes[i] = in planes[perm[i]] (7 planes: x,y,z,ux,uy,uz,active). */
+int pic_permute_particles(const PicParticles* in, PicParticles* out, const int32_t* perm,
+                          int64_t n, void* stream);
+
+/* ---- electrostatics (solvers/electrostatic_yee.py) ---------------------- */
+int64_t pic_cg_workspace_bytes(const PicDesc* desc);
+/* solve_poisson_with_conjugate_gradient (electrostatic_yee.py:71-156) with
+ * FP32 vectors and FP64 dot products; stops when r.r <= max(tol, rtol*|r0|)^2
+ * or after max_iter iterations.  phi in/out.  Synchronises the host once per
+ * 32 iterations to read the convergence flag.  iterations/residual optional. */
+int pic_poisson_cg(const PicDesc* desc, const float* rho, float* phi, double tol, double rtol,
+                   int32_t max_iter, void* workspace, int64_t workspace_bytes,
+                   int32_t* iterations, double* residual_norm, void* stream);
+/* _centered_tiled_electrostatic_gradient (electrostatic_yee.py:212-246):
+ * E interior = -centred grad phi, ghosts refreshed (phi must be refreshed). */
+int pic_efield_from_phi(const PicDesc* desc, const float* phi, float* const E[3], void* stream);
+
+/* ---- diagnostics (utils.py:108-187) ------------------------------------ */
+int64_t pic_energy_workspace_bytes(const PicDesc* desc, const PicSpecies* sp);
+/* out[3] (device, FP64) = electric, magnetic, kinetic energy, accumulated in FP64. */
+int pic_energy(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+               const float* const E[3], const float* const B[3], double* out,
+               void* workspace, int64_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYPIC3D_AMD_H */

@@ -1,0 +1,137 @@
+"""ctypes binding of libpypic3d_hip.so (C ABI declared in include/pypic3d_amd.h).
+
+The library is loaded lazily, after ``torch`` (so it binds to the HIP runtime
+torch already loaded; both carry the SONAME libamdhip64.so.7).  There is no
+fallback: if the library is missing or no GPU is present, calls raise.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpypic3d_hip.so")
+MAX_SPECIES = 16
+ABI_VERSION = 1
+
+c_int = ctypes.c_int
+c_i32 = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_dbl = ctypes.c_double
+c_vp = ctypes.c_void_p
+P3 = c_vp * 3
+
+
+class PicDesc(ctypes.Structure):
+    _fields_ = [
+        ("n", c_i32 * 3), ("g", c_i32), ("dims", c_i32 * 3), ("shape_factor", c_i32),
+        ("relativistic", c_i32), ("bc", c_i32 * 3), ("pbc", c_i32 * 3), ("pad0", c_i32),
+        ("d", c_dbl * 3), ("wind", c_dbl * 3), ("org_c", c_dbl * 3), ("sp_c", c_dbl * 3),
+        ("org_v", c_dbl * 3), ("sp_v", c_dbl * 3),
+        ("dt", c_dbl), ("C", c_dbl), ("eps", c_dbl), ("mu", c_dbl), ("alpha", c_dbl),
+    ]
+
+
+class PicSpecies(ctypes.Structure):
+    _fields_ = [
+        ("count", c_i32), ("slots", c_i32),
+        ("charge", c_dbl * MAX_SPECIES), ("mass", c_dbl * MAX_SPECIES), ("weight", c_dbl * MAX_SPECIES),
+        ("update_x", (ctypes.c_uint8 * 3) * MAX_SPECIES), ("update_u", (ctypes.c_uint8 * 3) * MAX_SPECIES),
+    ]
+
+
+class PicParticles(ctypes.Structure):
+    _fields_ = [("x", c_vp * 3), ("u", c_vp * 3), ("active", c_vp)]
+
+
+DP = ctypes.POINTER(PicDesc)
+SP = ctypes.POINTER(PicSpecies)
+PP = ctypes.POINTER(PicParticles)
+
+# name -> (restype, argtypes); must match include/pypic3d_amd.h
+SIGNATURES = {
+    "pic_abi_version": (c_int, []),
+    "pic_fill_f32": (c_int, [c_vp, c_i64, ctypes.c_float, c_vp]),
+    "pic_halo_refresh": (c_int, [DP, c_vp, c_int, c_vp]),
+    "pic_halo_fold": (c_int, [DP, c_vp, c_int, c_vp]),
+    "pic_zero_boundary": (c_int, [DP, c_vp, c_int, c_vp]),
+    "pic_constant_boundary": (c_int, [DP, c_vp, c_int, c_vp]),
+    "pic_filter_digital": (c_int, [DP, c_vp, c_vp, c_dbl, c_vp]),
+    "pic_filter_bilinear": (c_int, [DP, c_vp, c_vp, c_vp]),
+    "pic_yee_update_b": (c_int, [DP, P3, P3, P3, c_vp]),
+    "pic_yee_update_e": (c_int, [DP, P3, P3, P3, P3, c_vp]),
+    "pic_push": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, P3, c_vp]),
+    "pic_advance_positions": (c_int, [DP, SP, PP, c_dbl, P3, c_vp]),
+    "pic_particle_boundaries": (c_int, [DP, SP, PP, c_vp]),
+    "pic_deposit_esirkepov": (c_int, [DP, SP, PP, P3, c_vp]),
+    "pic_deposit_direct": (c_int, [DP, SP, PP, P3, c_vp]),
+    "pic_deposit_rho": (c_int, [DP, SP, PP, c_vp, c_vp]),
+    "pic_fused_push_esirkepov": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, PP, P3, c_vp]),
+    "pic_cell_keys": (c_int, [DP, SP, PP, c_vp, c_vp]),
+    "pic_sort_workspace_bytes": (c_i64, [c_i64]),
+    "pic_sort_permutation": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i32, c_vp, c_i64, c_vp]),
+    "pic_permute_particles": (c_int, [PP, PP, c_vp, c_i64, c_vp]),
+    "pic_cg_workspace_bytes": (c_i64, [DP]),
+    "pic_poisson_cg": (c_int, [DP, c_vp, c_vp, c_dbl, c_dbl, c_i32, c_vp, c_i64,
+                               ctypes.POINTER(c_i32), ctypes.POINTER(c_dbl), c_vp]),
+    "pic_efield_from_phi": (c_int, [DP, c_vp, P3, c_vp]),
+    "pic_energy_workspace_bytes": (c_i64, [DP, SP]),
+    "pic_energy": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, c_i64, c_vp]),
+}
+
+_lib = None
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH):
+    """Load (once) and type the C ABI.  Raises if the library is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise NativeLibraryMissing(
+            f"{path} is missing: build it with `python -m pypic3d_amd._build` "
+            "(there is no CPU fallback for the PIC hot path)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.pic_abi_version() != ABI_VERSION:
+        raise NativeLibraryMissing(f"ABI version mismatch: library {lib.pic_abi_version()} != {ABI_VERSION}")
+    _lib = lib
+    return lib
+
+
+def lib():
+    return _lib if _lib is not None else load()
+
+
+def check(rc: int, what: str):
+    if rc == 0:
+        return
+    if rc < 0:
+        raise ValueError(f"{what}: invalid argument or shape (code {rc})")
+    raise RuntimeError(f"{what}: HIP error {rc}")
+
+
+def call(name: str, *args):
+    check(getattr(lib(), name)(*args), name)
+
+
+def stream_handle():
+    return c_vp(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    return None if t is None else c_vp(t.data_ptr())
+
+
+def p3(ts):
+    return P3(*[t.data_ptr() for t in ts])

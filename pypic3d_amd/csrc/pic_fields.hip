@@ -1,0 +1,427 @@
+// Field kernels: fill, ghost refresh/fold, conducting walls, filters, Yee
+// curl updates and the electrostatic centred gradient.
+//
+// The Yee kernels read neighbours through ghost_source(), i.e. they see the
+// values a freshly refreshed copy of the input would hold
+// (first_order_yee.py:31, :115 refresh before differencing), so the caller's
+// input arrays are never modified -- the reference is functional.
+#include "pic_common.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ void fill_kernel(float* __restrict__ p, int64_t n, float v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+// Decompose a ghost-slab element id along `axis` into (slab index l, a, b),
+// with (a, b) the other two coordinates in increasing axis order.
+struct SlabCoord {
+    int l, a, b;
+};
+
+__device__ inline int64_t slab_to_index(const PicDesc& d, int axis, int pos, int a, int b) {
+    if (axis == 0) return fidx(d, pos, a, b);
+    if (axis == 1) return fidx(d, a, pos, b);
+    return fidx(d, a, b, pos);
+}
+
+__device__ inline void other_dims(const PicDesc& d, int axis, int& da, int& db) {
+    da = d.dims[axis == 0 ? 1 : 0];
+    db = d.dims[axis == 2 ? 1 : 2];
+}
+
+// ghost_cells.py:142-196: one axis of the refresh, for up to three fields.
+__global__ void refresh_axis_kernel(PicDesc d, float* f0, float* f1, float* f2, int axis, int mode) {
+    int g = d.g, n = d.n[axis];
+    int da, db;
+    other_dims(d, axis, da, db);
+    int64_t per = (int64_t)2 * g * da * db;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per * 3) return;
+    int which = (int)(t / per);
+    int64_t r = t - which * per;
+    float* f = which == 0 ? f0 : (which == 1 ? f1 : f2);
+    if (f == nullptr) return;
+    int b = (int)(r % db);
+    r /= db;
+    int a = (int)(r % da);
+    int l = (int)(r / da);
+    int dst = l < g ? l : n + l;          // upper ghosts n+g .. n+2g-1
+    float v = 0.0f;
+    if (mode == MODE_REDUCED_COPY) {
+        v = f[slab_to_index(d, axis, g, a, b)];
+    } else if (mode == MODE_PERIODIC) {
+        int src = l < g ? n + l : l;      // lower l <- n+l ; upper (l-g) <- g+(l-g)
+        v = f[slab_to_index(d, axis, src, a, b)];
+    }
+    f[slab_to_index(d, axis, dst, a, b)] = v;
+}
+
+// ghost_cells.py:218-289: one axis of the fold (non-reduced axis).
+// bc: 0 periodic self-exchange, 1 conducting reflection, other: dropped.
+__global__ void fold_axis_kernel(PicDesc d, float* f, int axis, int bc) {
+    int g = d.g, n = d.n[axis];
+    int da, db;
+    other_dims(d, axis, da, db);
+    int64_t per = (int64_t)g * da * db;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per) return;
+    int b = (int)(t % db);
+    int64_t r = t / db;
+    int a = (int)(r % da);
+    int l = (int)(r / da);
+    int64_t lo_i = slab_to_index(d, axis, l, a, b);
+    int64_t hi_i = slab_to_index(d, axis, n + g + l, a, b);
+    float lo = f[lo_i];
+    float hi = f[hi_i];
+    f[lo_i] = 0.0f;
+    f[hi_i] = 0.0f;
+    if (bc == 0) {
+        atomic_add_f32(&f[slab_to_index(d, axis, n + l, a, b)], lo);
+        atomic_add_f32(&f[slab_to_index(d, axis, g + l, a, b)], hi);
+    } else if (bc == 1) {
+        atomic_add_f32(&f[slab_to_index(d, axis, g + l, a, b)], -lo);
+        atomic_add_f32(&f[slab_to_index(d, axis, n + l, a, b)], -hi);
+    }
+}
+
+// ghost_cells.py:218-235: fold of a reduced (single-cell) axis.
+__global__ void fold_reduced_kernel(PicDesc d, float* f, int axis, int bc) {
+    int g = d.g;
+    int da, db;
+    other_dims(d, axis, da, db);
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)da * db) return;
+    int b = (int)(t % db);
+    int a = (int)(t / db);
+    float lower = 0.0f, upper = 0.0f;
+    for (int l = 0; l < g; ++l) {
+        int64_t i0 = slab_to_index(d, axis, l, a, b);
+        int64_t i1 = slab_to_index(d, axis, 1 + g + l, a, b);
+        lower += f[i0];
+        upper += f[i1];
+        f[i0] = 0.0f;
+        f[i1] = 0.0f;
+    }
+    float s = lower + upper;
+    int64_t c = slab_to_index(d, axis, g, a, b);
+    if (bc == 0) f[c] += s;
+    else if (bc == 1) f[c] += -s;
+}
+
+// Zero the wall planes g and n+g-1 of one axis (ghost_cells.py:344-362).
+__global__ void zero_planes_kernel(PicDesc d, float* f, int axis) {
+    int da, db;
+    other_dims(d, axis, da, db);
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)2 * da * db) return;
+    int which = (int)(t / ((int64_t)da * db));
+    int64_t r = t - (int64_t)which * da * db;
+    int b = (int)(r % db);
+    int a = (int)(r / db);
+    int pos = which == 0 ? d.g : d.n[axis] + d.g - 1;
+    f[slab_to_index(d, axis, pos, a, b)] = 0.0f;
+}
+
+// Fill the ghosts of one axis from the adjacent interior plane
+// (ghost_cells.py:365-386).
+__global__ void constant_ghosts_kernel(PicDesc d, float* f, int axis) {
+    int g = d.g, n = d.n[axis];
+    int da, db;
+    other_dims(d, axis, da, db);
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)2 * g * da * db) return;
+    int b = (int)(t % db);
+    int64_t r = t / db;
+    int a = (int)(r % da);
+    int l = (int)(r / da);
+    int dst = l < g ? l : n + l;
+    int src = l < g ? g : n + g - 1;
+    f[slab_to_index(d, axis, dst, a, b)] = f[slab_to_index(d, axis, src, a, b)];
+}
+
+// filters.py:234-291: interior <- 3x3x3 correlation; ghosts copied.
+template <bool kBilinear>
+__global__ void filter_kernel(PicDesc d, const float* __restrict__ in, float* __restrict__ out, float alpha) {
+    int64_t total = field_size(d);
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int k = (int)(t % d.dims[2]);
+    int64_t r = t / d.dims[2];
+    int j = (int)(r % d.dims[1]);
+    int i = (int)(r / d.dims[1]);
+    int g = d.g;
+    bool inside = i >= g && i < d.n[0] + g && j >= g && j < d.n[1] + g && k >= g && k < d.n[2] + g;
+    if (!inside) {
+        out[t] = in[t];
+        return;
+    }
+    const int64_t sx = (int64_t)d.dims[1] * d.dims[2], sy = d.dims[2];
+    if (kBilinear) {
+        const float w1[3] = {1.0f, 2.0f, 1.0f};
+        float acc = 0.0f;
+        for (int a = -1; a <= 1; ++a)
+            for (int b = -1; b <= 1; ++b)
+                for (int c = -1; c <= 1; ++c)
+                    acc += (w1[a + 1] * w1[b + 1] * w1[c + 1] / 64.0f) * in[t + a * sx + b * sy + c];
+        out[t] = acc;
+    } else {
+        float nw = (1.0f - alpha) / 6.0f;
+        float faces = in[t - sx] + in[t + sx] + in[t - sy] + in[t + sy] + in[t - 1] + in[t + 1];
+        out[t] = alpha * in[t] + nw * faces;
+    }
+}
+
+// Value of a refreshed copy of f at (i,j,k) where exactly one coordinate may
+// sit one cell outside the interior along `axis`.
+__device__ inline float refreshed_at(const float* __restrict__ f, const PicDesc& d, const int mode[3], int i,
+                                     int j, int k, int axis) {
+    int c[3] = {i, j, k};
+    int src = ghost_source(c[axis], d.n[axis], d.g, mode[axis]);
+    if (src < 0) return 0.0f;
+    c[axis] = src;
+    return f[fidx(d, c[0], c[1], c[2])];
+}
+
+// first_order_yee.py:96-142: B_out = B_in - (dt/2) curl E (forward diffs).
+__global__ void yee_b_kernel(PicDesc d, const float* __restrict__ Ex, const float* __restrict__ Ey,
+                             const float* __restrict__ Ez, const float* __restrict__ Bx,
+                             const float* __restrict__ By, const float* __restrict__ Bz, float* __restrict__ ox,
+                             float* __restrict__ oy, float* __restrict__ oz, float hdt, float idx_, float idy,
+                             float idz, int m0, int m1, int m2) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    int i = blockIdx.z;
+    if (k >= d.n[2]) return;
+    i += d.g;
+    j += d.g;
+    k += d.g;
+    const int mode[3] = {m0, m1, m2};
+    int64_t c = fidx(d, i, j, k);
+    float ex = Ex[c], ey = Ey[c], ez = Ez[c];
+    float ez_yp = refreshed_at(Ez, d, mode, i, j + 1, k, 1);
+    float ey_zp = refreshed_at(Ey, d, mode, i, j, k + 1, 2);
+    float ex_zp = refreshed_at(Ex, d, mode, i, j, k + 1, 2);
+    float ez_xp = refreshed_at(Ez, d, mode, i + 1, j, k, 0);
+    float ey_xp = refreshed_at(Ey, d, mode, i + 1, j, k, 0);
+    float ex_yp = refreshed_at(Ex, d, mode, i, j + 1, k, 1);
+    float curl_x = (ez_yp - ez) * idy - (ey_zp - ey) * idz;
+    float curl_y = (ex_zp - ex) * idz - (ez_xp - ez) * idx_;
+    float curl_z = (ey_xp - ey) * idx_ - (ex_yp - ex) * idy;
+    ox[c] = Bx[c] - hdt * curl_x;
+    oy[c] = By[c] - hdt * curl_y;
+    oz[c] = Bz[c] - hdt * curl_z;
+}
+
+// first_order_yee.py:12-72: E_out = E_in + dt (C^2 curl B - J/eps) (backward diffs).
+__global__ void yee_e_kernel(PicDesc d, const float* __restrict__ Ex, const float* __restrict__ Ey,
+                             const float* __restrict__ Ez, const float* __restrict__ Bx,
+                             const float* __restrict__ By, const float* __restrict__ Bz,
+                             const float* __restrict__ Jx, const float* __restrict__ Jy,
+                             const float* __restrict__ Jz, float* __restrict__ ox, float* __restrict__ oy,
+                             float* __restrict__ oz, float dt, float c2, float inv_eps, float idx_, float idy,
+                             float idz, int m0, int m1, int m2) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    int i = blockIdx.z;
+    if (k >= d.n[2]) return;
+    i += d.g;
+    j += d.g;
+    k += d.g;
+    const int mode[3] = {m0, m1, m2};
+    int64_t c = fidx(d, i, j, k);
+    float bx = Bx[c], by = By[c], bz = Bz[c];
+    float bz_ym = refreshed_at(Bz, d, mode, i, j - 1, k, 1);
+    float by_zm = refreshed_at(By, d, mode, i, j, k - 1, 2);
+    float bx_zm = refreshed_at(Bx, d, mode, i, j, k - 1, 2);
+    float bz_xm = refreshed_at(Bz, d, mode, i - 1, j, k, 0);
+    float by_xm = refreshed_at(By, d, mode, i - 1, j, k, 0);
+    float bx_ym = refreshed_at(Bx, d, mode, i, j - 1, k, 1);
+    float curl_x = (bz - bz_ym) * idy - (by - by_zm) * idz;
+    float curl_y = (bx - bx_zm) * idz - (bz - bz_xm) * idx_;
+    float curl_z = (by - by_xm) * idx_ - (bx - bx_ym) * idy;
+    ox[c] = Ex[c] + (c2 * curl_x - Jx[c] * inv_eps) * dt;
+    oy[c] = Ey[c] + (c2 * curl_y - Jy[c] * inv_eps) * dt;
+    oz[c] = Ez[c] + (c2 * curl_z - Jz[c] * inv_eps) * dt;
+}
+
+// electrostatic_yee.py:212-246: E interior = -(phi[+1]-phi[-1])/(2d), reading
+// phi's own (already boundary-applied) ghosts.
+__global__ void efield_from_phi_kernel(PicDesc d, const float* __restrict__ phi, float* __restrict__ ex,
+                                       float* __restrict__ ey, float* __restrict__ ez, float h2x, float h2y,
+                                       float h2z) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    int i = blockIdx.z;
+    if (k >= d.n[2]) return;
+    i += d.g;
+    j += d.g;
+    k += d.g;
+    int64_t c = fidx(d, i, j, k);
+    const int64_t sx = (int64_t)d.dims[1] * d.dims[2], sy = d.dims[2];
+    ex[c] = -1.0f * (phi[c + sx] - phi[c - sx]) * h2x;
+    ey[c] = -1.0f * (phi[c + sy] - phi[c - sy]) * h2y;
+    ez[c] = -1.0f * (phi[c + 1] - phi[c - 1]) * h2z;
+}
+
+bool desc_ok(const PicDesc* d) {
+    if (!d) return false;
+    if (d->g < 1) return false;
+    for (int a = 0; a < 3; ++a) {
+        if (d->n[a] < 1 || d->dims[a] != d->n[a] + 2 * d->g) return false;
+        // a non-reduced axis narrower than the guard band would alias slabs
+        if (d->n[a] > 1 && d->n[a] < d->g) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+// Host helpers shared with other translation units.
+int pic_refresh_fields(const PicDesc* d, float* f0, float* f1, float* f2, int bc_type, hipStream_t s) {
+    for (int axis = 0; axis < 3; ++axis) {
+        int da = d->dims[axis == 0 ? 1 : 0], db = d->dims[axis == 2 ? 1 : 2];
+        int64_t work = (int64_t)2 * d->g * da * db * 3;
+        hipLaunchKernelGGL(refresh_axis_kernel, dim3(grid_1d(work, kBlock)), dim3(kBlock), 0, s, *d, f0, f1, f2,
+                           axis, refresh_mode(*d, axis, bc_type));
+        PIC_CHECK_LAUNCH();
+    }
+    return 0;
+}
+
+static dim3 interior_grid(const PicDesc* d, int block) {
+    return dim3((unsigned)((d->n[2] + block - 1) / block), (unsigned)d->n[1], (unsigned)d->n[0]);
+}
+
+static int interior_block(const PicDesc* d) { return d->n[2] >= 128 ? 128 : 64; }
+
+extern "C" {
+
+int pic_abi_version(void) { return PIC_ABI_VERSION; }
+
+int pic_fill_f32(float* ptr, int64_t n, float value, void* stream) {
+    PIC_REQUIRE(ptr != nullptr && n >= 0);
+    if (n == 0) return 0;
+    int blocks = grid_1d(n, kBlock);
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(kBlock), 0, as_stream(stream), ptr, n, value);
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+int pic_halo_refresh(const PicDesc* desc, float* field, int bc_type, void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && field && (bc_type == 0 || bc_type == 1));
+    return pic_refresh_fields(desc, field, nullptr, nullptr, bc_type, as_stream(stream));
+}
+
+int pic_halo_fold(const PicDesc* desc, float* field, int bc_type, void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && field && (bc_type == 0 || bc_type == 1));
+    hipStream_t s = as_stream(stream);
+    for (int axis = 0; axis < 3; ++axis) {
+        int bc = bc_type == 0 ? desc->bc[axis] : desc->pbc[axis];
+        int da = desc->dims[axis == 0 ? 1 : 0], db = desc->dims[axis == 2 ? 1 : 2];
+        if (desc->n[axis] == 1) {
+            hipLaunchKernelGGL(fold_reduced_kernel, dim3(grid_1d((int64_t)da * db, kBlock)), dim3(kBlock), 0, s,
+                               *desc, field, axis, bc);
+        } else {
+            hipLaunchKernelGGL(fold_axis_kernel, dim3(grid_1d((int64_t)desc->g * da * db, kBlock)), dim3(kBlock),
+                               0, s, *desc, field, axis, bc);
+        }
+        PIC_CHECK_LAUNCH();
+    }
+    return 0;
+}
+
+int pic_zero_boundary(const PicDesc* desc, float* field, int axis, void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && field && axis >= 0 && axis < 3);
+    hipStream_t s = as_stream(stream);
+    if (desc->bc[axis] == 1) {
+        int da = desc->dims[axis == 0 ? 1 : 0], db = desc->dims[axis == 2 ? 1 : 2];
+        hipLaunchKernelGGL(zero_planes_kernel, dim3(grid_1d((int64_t)2 * da * db, kBlock)), dim3(kBlock), 0, s,
+                           *desc, field, axis);
+        PIC_CHECK_LAUNCH();
+    }
+    return pic_refresh_fields(desc, field, nullptr, nullptr, 0, s);
+}
+
+int pic_constant_boundary(const PicDesc* desc, float* field, int axis, void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && field && axis >= 0 && axis < 3);
+    hipStream_t s = as_stream(stream);
+    int rc = pic_refresh_fields(desc, field, nullptr, nullptr, 0, s);
+    if (rc) return rc;
+    if (desc->bc[axis] == 1) {
+        int da = desc->dims[axis == 0 ? 1 : 0], db = desc->dims[axis == 2 ? 1 : 2];
+        hipLaunchKernelGGL(constant_ghosts_kernel, dim3(grid_1d((int64_t)2 * desc->g * da * db, kBlock)),
+                           dim3(kBlock), 0, s, *desc, field, axis);
+        PIC_CHECK_LAUNCH();
+    }
+    return 0;
+}
+
+int pic_filter_digital(const PicDesc* desc, const float* in, float* out, double alpha, void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && in && out && in != out);
+    int64_t n = field_size(*desc);
+    hipLaunchKernelGGL(filter_kernel<false>, dim3(grid_1d(n, kBlock)), dim3(kBlock), 0, as_stream(stream), *desc, in,
+                       out, (float)alpha);
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+int pic_filter_bilinear(const PicDesc* desc, const float* in, float* out, void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && in && out && in != out);
+    int64_t n = field_size(*desc);
+    hipLaunchKernelGGL(filter_kernel<true>, dim3(grid_1d(n, kBlock)), dim3(kBlock), 0, as_stream(stream), *desc, in,
+                       out, 1.0f);
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+int pic_yee_update_b(const PicDesc* desc, const float* const E[3], const float* const B_in[3], float* const B_out[3],
+                     void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && E && B_in && B_out);
+    for (int c = 0; c < 3; ++c) PIC_REQUIRE(E[c] && B_in[c] && B_out[c]);
+    for (int c = 0; c < 3; ++c)
+        for (int e = 0; e < 3; ++e) PIC_REQUIRE((const float*)B_out[c] != E[e]);
+    hipStream_t s = as_stream(stream);
+    int block = interior_block(desc);
+    hipLaunchKernelGGL(yee_b_kernel, interior_grid(desc, block), dim3(block), 0, s, *desc, E[0], E[1], E[2],
+                       B_in[0], B_in[1], B_in[2], B_out[0], B_out[1], B_out[2], (float)(desc->dt / 2),
+                       (float)(1.0 / desc->d[0]), (float)(1.0 / desc->d[1]), (float)(1.0 / desc->d[2]),
+                       refresh_mode(*desc, 0, 0), refresh_mode(*desc, 1, 0), refresh_mode(*desc, 2, 0));
+    PIC_CHECK_LAUNCH();
+    return pic_refresh_fields(desc, B_out[0], B_out[1], B_out[2], 0, s);
+}
+
+int pic_yee_update_e(const PicDesc* desc, const float* const E_in[3], const float* const B[3], const float* const J[3],
+                     float* const E_out[3], void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && E_in && B && J && E_out);
+    for (int c = 0; c < 3; ++c) PIC_REQUIRE(E_in[c] && B[c] && J[c] && E_out[c]);
+    for (int c = 0; c < 3; ++c)
+        for (int e = 0; e < 3; ++e) PIC_REQUIRE((const float*)E_out[c] != B[e] && (const float*)E_out[c] != J[e]);
+    hipStream_t s = as_stream(stream);
+    int block = interior_block(desc);
+    hipLaunchKernelGGL(yee_e_kernel, interior_grid(desc, block), dim3(block), 0, s, *desc, E_in[0], E_in[1], E_in[2],
+                       B[0], B[1], B[2], J[0], J[1], J[2], E_out[0], E_out[1], E_out[2], (float)desc->dt,
+                       (float)(desc->C * desc->C), (float)(1.0 / desc->eps), (float)(1.0 / desc->d[0]),
+                       (float)(1.0 / desc->d[1]), (float)(1.0 / desc->d[2]), refresh_mode(*desc, 0, 0),
+                       refresh_mode(*desc, 1, 0), refresh_mode(*desc, 2, 0));
+    PIC_CHECK_LAUNCH();
+    return pic_refresh_fields(desc, E_out[0], E_out[1], E_out[2], 0, s);
+}
+
+int pic_efield_from_phi(const PicDesc* desc, const float* phi, float* const E[3], void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && phi && E && E[0] && E[1] && E[2]);
+    hipStream_t s = as_stream(stream);
+    int block = interior_block(desc);
+    hipLaunchKernelGGL(efield_from_phi_kernel, interior_grid(desc, block), dim3(block), 0, s, *desc, phi, E[0], E[1],
+                       E[2], (float)(1.0 / (2.0 * desc->d[0])), (float)(1.0 / (2.0 * desc->d[1])),
+                       (float)(1.0 / (2.0 * desc->d[2])));
+    PIC_CHECK_LAUNCH();
+    return pic_refresh_fields(desc, E[0], E[1], E[2], 0, s);
+}
+
+}  // extern "C"
