@@ -73,6 +73,15 @@ __device__ inline int anchor_of(double pos, double org, double sp, int sf) {
     return (int)t;
 }
 
+// Node coordinate anchor*spacing + origin rounded like the FP64 reference
+// (product and sum rounded separately, no fused multiply-add).
+__device__ inline double node_coord(int a, double sp, double org) {
+    return __dadd_rn(__dmul_rn((double)a, sp), org);
+}
+
+// x + v*dt rounded like the FP64 reference (update_tiled_particle_positions).
+__device__ inline double step_pos(double x, double v, double dt) { return __dadd_rn(x, __dmul_rn(v, dt)); }
+
 // shapes.py:5-54 -- weights of stencil points (-1, 0, +1) for r = delta / d.
 __device__ inline void shape3(double r, int sf, double w[3]) {
     if (sf == 1) {

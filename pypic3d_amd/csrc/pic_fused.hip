@@ -1,0 +1,285 @@
+// Fused particle phase of the electrodynamic Esirkepov step
+// (evolve.py:36-43 push, :70-79 deposit + position update + refresh).
+//
+// One block owns kPerBlock consecutive particle slots, processed in rounds of
+// one slot per thread.  Per particle: gather E+E_ext / B+B_ext on the Yee
+// staggered grids, Boris push, then the Esirkepov deposit.
+//
+// Deposit organisation.  With cell-sorted particles the 64 lanes of a wave
+// cover two or three cells, so per-lane global atomics would hammer the same
+// twelve addresses.  The common case -- 3-D, first-order shape, particle
+// stays in its cell on every axis -- deposits exactly 12 values at fixed
+// offsets from its cell (Jx at (a, b..b+1, c..c+1), Jy at (a..a+1, b,
+// c..c+1), Jz at (a..a+1, b..b+1, c)).  Those lanes are combined by a wave
+// segmented inclusive scan over runs of equal cell; only the last lane of
+// each run issues the 12 atomics.  Every other particle (a face crossing,
+// second-order shape, reduced axes) is put on a per-block LDS queue and
+// deposited after the rounds by the general 5-slot path, with the whole
+// block working on the queue so there is no divergence cost in the rounds.
+//
+// Anchors use an FP32 estimate and fall back to the exact FP64 division of
+// the reference whenever the estimate is within a margin of a cell face, so
+// cell indices still agree bit for bit with an FP64 evaluation.
+#include "pic_particle_math.h"
+
+namespace {
+
+using namespace pic;
+
+constexpr int kThreads = 256;
+constexpr int kRounds = 4;
+constexpr int kPerBlock = kThreads * kRounds;
+
+// Anchor from an FP32 estimate t of (x - origin)/spacing: taken as is when it
+// sits farther than `margin` from a face (floor) or a half-cell (round, TSC);
+// otherwise the exact FP64 division of the reference on x_exact decides.  The
+// margin covers the FP32 error of the estimate (a few ulp of |t|).
+__device__ inline int anchor_est(float t, float mag, double x_exact, double org, double sp, int sf) {
+    float tt = sf == 1 ? t : t + 0.5f;
+    float f = floorf(tt);
+    float fr = tt - f;
+    // FP32 error of the estimate <= a few ulp of (|x| + |origin|) / spacing
+    float margin = 1.0e-4f + 1.0e-6f * mag;
+    if (fr >= margin && fr <= 1.0f - margin) {
+        f = fminf(fmaxf(f, -1.0e8f), 1.0e8f);
+        return (int)f;
+    }
+    return anchor_of(x_exact, org, sp, sf);
+}
+
+__device__ inline int anchor_fast(float x, float org32, float isp32, double org, double sp, int sf) {
+    return anchor_est((x - org32) * isp32, (fabsf(x) + fabsf(org32)) * isp32, (double)x, org, sp, sf);
+}
+
+__device__ inline void shape3f(float r, int sf, float w[3]) {
+    if (sf == 1) {
+        w[0] = 0.0f;
+        w[1] = 1.0f - r;
+        w[2] = r;
+    } else {
+        w[0] = 0.5f * (0.5f - r) * (0.5f - r);
+        w[1] = 0.75f - r * r;
+        w[2] = 0.5f * (0.5f + r) * (0.5f + r);
+    }
+}
+
+struct AxisConst {
+    float org_c, org_v, isp_c, isp_v, inv_d;
+};
+
+// Gather stencil on one axis for one grid (center or vertex); also returns
+// the anchor and the FP64-exact offset from the anchor node.
+__device__ inline void gather_axis_fast(float x, float org32, float isp32, double org, double sp, int sf, int L,
+                                        bool active, int g, GatherAxis& s, int& anchor, float& delta) {
+    int a = anchor_fast(x, org32, isp32, org, sp, sf);
+    double dl = (double)x - node_coord(a, sp, org);
+    anchor = a;
+    delta = (float)dl;
+    float w[3];
+    shape3f((float)dl * isp32, sf, w);
+    if (active) {
+        for (int q = 0; q < 3; ++q) {
+            s.p[q] = pmod(a - 1 + q, L);
+            s.w[q] = w[q];
+        }
+        s.k0 = sf == 1 ? 1 : 0;
+        s.k1 = 3;
+    } else {
+        s.p[0] = g;
+        s.w[0] = w[0] + w[1] + w[2];
+        s.k0 = 0;
+        s.k1 = 1;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, PicSpecies sp, PicParticles in,
+                                                                    FieldPtrs fp, PicParticles out, float* Jx,
+                                                                    float* Jy, float* Jz, int64_t n) {
+    __shared__ int q_idx[kPerBlock];
+    __shared__ float q_v[3][kPerBlock];
+    __shared__ int q_n;
+    if (threadIdx.x == 0) q_n = 0;
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int sf = d.shape_factor;
+    const bool fast3d = sf == 1 && d.n[0] > 1 && d.n[1] > 1 && d.n[2] > 1;
+    AxisConst ac[3];
+    for (int a = 0; a < 3; ++a) {
+        ac[a].org_c = (float)d.org_c[a];
+        ac[a].org_v = (float)d.org_v[a];
+        ac[a].isp_c = (float)(1.0 / d.sp_c[a]);
+        ac[a].isp_v = (float)(1.0 / d.sp_v[a]);
+        ac[a].inv_d = (float)(1.0 / d.d[a]);
+    }
+    const int64_t sx = (int64_t)d.dims[1] * d.dims[2];
+    const int64_t sy = d.dims[2];
+    const float fdt = (float)d.dt;
+    const int64_t base = (int64_t)blockIdx.x * kPerBlock;
+
+    for (int r = 0; r < kRounds; ++r) {
+        const int local = r * kThreads + threadIdx.x;
+        const int64_t i = base + local;
+        const bool valid = i < n;
+        const int s = valid ? (int)(i / sp.slots) : 0;
+        bool act = valid && in.active[i] != 0;
+        int key = -1 - lane;
+        float val[12];
+        for (int m = 0; m < 12; ++m) val[m] = 0.0f;
+        int cell = 0;
+
+        if (act) {
+            float xf[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
+            float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
+            // ---- gather (boris.py:302-433) on center and vertex lines
+            GatherAxis gc[3], gv[3];
+            int anc[3], anv[3];
+            float dc[3], dv[3];
+            for (int a = 0; a < 3; ++a) {
+                bool axact = d.n[a] > 1;
+                gather_axis_fast(xf[a], ac[a].org_c, ac[a].isp_c, d.org_c[a], d.sp_c[a], sf, d.dims[a], axact, d.g,
+                                 gc[a], anc[a], dc[a]);
+                gather_axis_fast(xf[a], ac[a].org_v, ac[a].isp_v, d.org_v[a], d.sp_v[a], sf, d.dims[a], axact, d.g,
+                                 gv[a], anv[a], dv[a]);
+            }
+            float E[3], B[3];
+            E[0] = gather3(fp.E[0], fp.Ee[0], d, gv[0], gc[1], gc[2]);
+            E[1] = gather3(fp.E[1], fp.Ee[1], d, gc[0], gv[1], gc[2]);
+            E[2] = gather3(fp.E[2], fp.Ee[2], d, gc[0], gc[1], gv[2]);
+            B[0] = gather3(fp.B[0], fp.Be[0], d, gc[0], gv[1], gv[2]);
+            B[1] = gather3(fp.B[1], fp.Be[1], d, gv[0], gc[1], gv[2]);
+            B[2] = gather3(fp.B[2], fp.Be[2], d, gv[0], gv[1], gc[2]);
+            // ---- push (boris.py:14-124), masked by update_u (particle_push.py:134-142)
+            float nv[3] = {v[0], v[1], v[2]};
+            float h = (float)(sp.charge[s] * d.dt / (2.0 * sp.mass[s]));
+            boris_push(nv, E, B, h, d.relativistic, (float)d.C);
+            for (int a = 0; a < 3; ++a)
+                if (sp.update_u[s][a]) v[a] = nv[a];
+            // ---- new position (Esirkepov.py:125-127) and regular-case test
+            double xn[3];
+            bool regular = fast3d;
+            float S0[3][2], D[3];
+            for (int a = 0; a < 3; ++a) {
+                bool ux = sp.update_x[s][a] != 0;
+                xn[a] = ux ? step_pos((double)xf[a], (double)v[a], d.dt) : (double)xf[a];
+                if (!regular) continue;
+                float dvt = ux ? v[a] * fdt : 0.0f;
+                int an = anchor_est((xf[a] + dvt - ac[a].org_c) * ac[a].isp_c,
+                                    (fabsf(xf[a]) + fabsf(dvt) + fabsf(ac[a].org_c)) * ac[a].isp_c, xn[a],
+                                    d.org_c[a], d.sp_c[a], 1);
+                regular = regular && an == anc[a] && anc[a] >= 0 && anc[a] + 1 < d.dims[a];
+                float r0 = dc[a] * ac[a].inv_d;
+                S0[a][0] = 1.0f - r0;
+                S0[a][1] = r0;
+                D[a] = dvt * ac[a].inv_d;      // S_new - S_old = (+D at a+1, -D at a)
+            }
+            if (regular) {
+                // Esirkepov weights in difference form with D = (-D, +D) at (a, a+1):
+                // W = D_c * (S0 S0 + (S0 D + D S0)/2 + D D/3) over the other two axes,
+                // J_c = -(q/(d_o1 d_o2 dt)) * cumsum_c(W) -> value at slot a only.
+                double qw = sp.charge[s] * sp.weight[s];
+                for (int c = 0; c < 3; ++c) {
+                    int o1 = c == 0 ? 1 : 0, o2 = c == 2 ? 1 : 2;
+                    float pre = (float)(-(qw / (d.d[o1] * d.d[o2])) / d.dt) * (-D[c]);
+                    for (int j = 0; j < 2; ++j) {
+                        float dj = j ? D[o1] : -D[o1];
+                        for (int k = 0; k < 2; ++k) {
+                            float dk = k ? D[o2] : -D[o2];
+                            float y = S0[o1][j] * S0[o2][k] + 0.5f * (S0[o1][j] * dk + dj * S0[o2][k]) +
+                                      (1.0f / 3.0f) * dj * dk;
+                            val[c * 4 + j * 2 + k] = pre * y;
+                        }
+                    }
+                }
+                cell = (int)fidx(d, anc[0], anc[1], anc[2]);
+                key = cell;
+            } else {
+                int slot = atomicAdd(&q_n, 1);
+                q_idx[slot] = local;
+                q_v[0][slot] = v[0];
+                q_v[1][slot] = v[1];
+                q_v[2][slot] = v[2];
+            }
+            // ---- position update + particle boundaries (particle_tile_communication.py)
+            for (int a = 0; a < 3; ++a) apply_axis_boundary(d.pbc[a], d.wind[a], xn[a], v[a], act);
+            for (int a = 0; a < 3; ++a) {
+                out.x[a][i] = act ? (float)xn[a] : 0.0f;
+                out.u[a][i] = act ? v[a] : 0.0f;
+            }
+            out.active[i] = act ? 1 : 0;
+        } else if (valid) {
+            for (int a = 0; a < 3; ++a) {
+                out.x[a][i] = 0.0f;
+                out.u[a][i] = 0.0f;
+            }
+            out.active[i] = 0;
+        }
+
+        // ---- wave segmented inclusive scan over runs of equal cell
+        unsigned long long any_regular = __ballot(key >= 0);
+        if (any_regular) {
+            int prev = __shfl_up(key, 1);
+            bool head = lane == 0 || prev != key;
+            unsigned long long heads = __ballot(head);
+            unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+            int seg = __popcll(heads & upto);
+            for (int dd = 1; dd < 64; dd <<= 1) {
+                int so = __shfl_up(seg, dd);
+                bool same = lane >= dd && so == seg;
+                if (__ballot(same) == 0ull) break;
+                for (int m = 0; m < 12; ++m) {
+                    float o = __shfl_up(val[m], dd);
+                    if (same) val[m] += o;
+                }
+            }
+            bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+            if (tail && key >= 0) {
+                const int64_t c0 = cell;
+                const int64_t offx[4] = {0, 1, sy, sy + 1};       // (j,k) over (y,z)
+                const int64_t offy[4] = {0, 1, sx, sx + 1};       // (i,k) over (x,z)
+                const int64_t offz[4] = {0, sy, sx, sx + sy};     // (i,j) over (x,y)
+                for (int m = 0; m < 4; ++m) {
+                    if (val[m] != 0.0f) atomic_add_f32(&Jx[c0 + offx[m]], val[m]);
+                    if (val[4 + m] != 0.0f) atomic_add_f32(&Jy[c0 + offy[m]], val[4 + m]);
+                    if (val[8 + m] != 0.0f) atomic_add_f32(&Jz[c0 + offz[m]], val[8 + m]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- general path for queued particles (Esirkepov.py:105-331)
+    const int nq = q_n;
+    float* J[3] = {Jx, Jy, Jz};
+    for (int q = threadIdx.x; q < nq; q += kThreads) {
+        int64_t i = base + q_idx[q];
+        int s = (int)(i / sp.slots);
+        float v[3] = {q_v[0][q], q_v[1][q], q_v[2][q]};
+        double xo[3], xn[3];
+        load_pos(in, i, xo);
+        for (int a = 0; a < 3; ++a) xn[a] = sp.update_x[s][a] ? step_pos(xo[a], (double)v[a], d.dt) : xo[a];
+        esirkepov_particle(d, sp.charge[s] * sp.weight[s], xo, xn, v, J);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pic_fused_push_esirkepov(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                             const float* const E[3], const float* const B[3], const float* const E_ext[3],
+                             const float* const B_ext[3], PicParticles* out, float* const J[3], void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && particles_ok(sp, in) && particles_ok(sp, out) && E && B && J);
+    PIC_REQUIRE(desc->n[0] > 1 || desc->n[1] > 1 || desc->n[2] > 1);
+    for (int a = 0; a < 3; ++a) PIC_REQUIRE(E[a] && B[a] && J[a] && in->x[a] != out->x[a] && in->u[a] != out->u[a]);
+    PIC_REQUIRE(in->active != out->active);
+    PIC_REQUIRE(field_size(*desc) < ((int64_t)1 << 31));
+    int64_t n = (int64_t)sp->count * sp->slots;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(fused_push_esirkepov_v2, dim3(grid_1d(n, kPerBlock)), dim3(kThreads), 0, as_stream(stream),
+                       *desc, *sp, *in, make_fields(E, B, E_ext, B_ext), *out, J[0], J[1], J[2], n);
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+}  // extern "C"
