@@ -100,10 +100,22 @@ __device__ inline void atomic_add_f32(float* p, float v) {
     unsafeAtomicAdd(p, v);
 }
 
-// positive modulo (jnp.mod of an int)
+// positive modulo (jnp.mod of an int); the in-range case skips the division
 __device__ inline int pmod(int a, int L) {
+    if ((unsigned)a < (unsigned)L) return a;
     int r = a % L;
     return r < 0 ? r + L : r;
+}
+
+// species of flat slot i (slots are species-major, S <= PIC_MAX_SPECIES)
+__device__ inline int species_of(int64_t i, int slots, int count) {
+    int s = 0;
+    int64_t end = slots;
+    while (s + 1 < count && i >= end) {
+        ++s;
+        end += slots;
+    }
+    return s;
 }
 
 static inline int grid_1d(int64_t n, int block) {

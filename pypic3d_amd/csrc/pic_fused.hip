@@ -92,6 +92,50 @@ __device__ inline void gather_axis_fast(float x, float org32, float isp32, doubl
     }
 }
 
+// First-order (CIC) stencil on an active axis: points a, a+1 (the weight of
+// a-1 is exactly zero, shapes.py:46-48), wrapped into the ghosted line like
+// jnp.mod in prepare_particle_axis_stencil (grid_and_stencil.py:141-166).
+struct CicAxis {
+    int p0, p1;
+    float w0, w1;
+};
+
+__device__ inline void cic_axis(float x, float org32, float isp32, double org, double sp, int L, CicAxis& s,
+                                int& anchor, float& delta) {
+    int a = anchor_fast(x, org32, isp32, org, sp, 1);
+    double dl = (double)x - node_coord(a, sp, org);
+    anchor = a;
+    delta = (float)dl;
+    float r = (float)dl * isp32;
+    s.p0 = pmod(a, L);
+    s.p1 = pmod(a + 1, L);
+    s.w0 = 1.0f - r;
+    s.w1 = r;
+}
+
+__device__ inline float gather_cic(const float* __restrict__ f, const float* __restrict__ fe, const PicDesc& d,
+                                   const CicAxis& ax, const CicAxis& ay, const CicAxis& az) {
+    const int64_t sy = d.dims[2], sx = (int64_t)d.dims[1] * d.dims[2];
+    const int64_t r00 = ax.p0 * sx + ay.p0 * sy, r01 = ax.p0 * sx + ay.p1 * sy;
+    const int64_t r10 = ax.p1 * sx + ay.p0 * sy, r11 = ax.p1 * sx + ay.p1 * sy;
+    float v000 = f[r00 + az.p0], v001 = f[r00 + az.p1], v010 = f[r01 + az.p0], v011 = f[r01 + az.p1];
+    float v100 = f[r10 + az.p0], v101 = f[r10 + az.p1], v110 = f[r11 + az.p0], v111 = f[r11 + az.p1];
+    if (fe) {
+        v000 += fe[r00 + az.p0];
+        v001 += fe[r00 + az.p1];
+        v010 += fe[r01 + az.p0];
+        v011 += fe[r01 + az.p1];
+        v100 += fe[r10 + az.p0];
+        v101 += fe[r10 + az.p1];
+        v110 += fe[r11 + az.p0];
+        v111 += fe[r11 + az.p1];
+    }
+    float c00 = v000 * az.w0 + v001 * az.w1, c01 = v010 * az.w0 + v011 * az.w1;
+    float c10 = v100 * az.w0 + v101 * az.w1, c11 = v110 * az.w0 + v111 * az.w1;
+    return ax.w0 * (ay.w0 * c00 + ay.w1 * c01) + ax.w1 * (ay.w0 * c10 + ay.w1 * c11);
+}
+
+template <bool kFast>
 __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, PicSpecies sp, PicParticles in,
                                                                     FieldPtrs fp, PicParticles out, float* Jx,
                                                                     float* Jy, float* Jz, int64_t n) {
@@ -103,7 +147,7 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
 
     const int lane = threadIdx.x & 63;
     const int sf = d.shape_factor;
-    const bool fast3d = sf == 1 && d.n[0] > 1 && d.n[1] > 1 && d.n[2] > 1;
+    const bool fast3d = kFast;
     AxisConst ac[3];
     for (int a = 0; a < 3; ++a) {
         ac[a].org_c = (float)d.org_c[a];
@@ -121,10 +165,11 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
         const int local = r * kThreads + threadIdx.x;
         const int64_t i = base + local;
         const bool valid = i < n;
-        const int s = valid ? (int)(i / sp.slots) : 0;
+        const int s = valid ? species_of(i, sp.slots, sp.count) : 0;
         bool act = valid && in.active[i] != 0;
         int key = -1 - lane;
         float val[12];
+#pragma unroll
         for (int m = 0; m < 12; ++m) val[m] = 0.0f;
         int cell = 0;
 
@@ -132,23 +177,38 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
             float xf[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
             float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
             // ---- gather (boris.py:302-433) on center and vertex lines
-            GatherAxis gc[3], gv[3];
             int anc[3], anv[3];
             float dc[3], dv[3];
-            for (int a = 0; a < 3; ++a) {
-                bool axact = d.n[a] > 1;
-                gather_axis_fast(xf[a], ac[a].org_c, ac[a].isp_c, d.org_c[a], d.sp_c[a], sf, d.dims[a], axact, d.g,
-                                 gc[a], anc[a], dc[a]);
-                gather_axis_fast(xf[a], ac[a].org_v, ac[a].isp_v, d.org_v[a], d.sp_v[a], sf, d.dims[a], axact, d.g,
-                                 gv[a], anv[a], dv[a]);
-            }
             float E[3], B[3];
-            E[0] = gather3(fp.E[0], fp.Ee[0], d, gv[0], gc[1], gc[2]);
-            E[1] = gather3(fp.E[1], fp.Ee[1], d, gc[0], gv[1], gc[2]);
-            E[2] = gather3(fp.E[2], fp.Ee[2], d, gc[0], gc[1], gv[2]);
-            B[0] = gather3(fp.B[0], fp.Be[0], d, gc[0], gv[1], gv[2]);
-            B[1] = gather3(fp.B[1], fp.Be[1], d, gv[0], gc[1], gv[2]);
-            B[2] = gather3(fp.B[2], fp.Be[2], d, gv[0], gv[1], gc[2]);
+            if constexpr (kFast) {
+                CicAxis gc[3], gv[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    cic_axis(xf[a], ac[a].org_c, ac[a].isp_c, d.org_c[a], d.sp_c[a], d.dims[a], gc[a], anc[a], dc[a]);
+                    cic_axis(xf[a], ac[a].org_v, ac[a].isp_v, d.org_v[a], d.sp_v[a], d.dims[a], gv[a], anv[a], dv[a]);
+                }
+                E[0] = gather_cic(fp.E[0], fp.Ee[0], d, gv[0], gc[1], gc[2]);
+                E[1] = gather_cic(fp.E[1], fp.Ee[1], d, gc[0], gv[1], gc[2]);
+                E[2] = gather_cic(fp.E[2], fp.Ee[2], d, gc[0], gc[1], gv[2]);
+                B[0] = gather_cic(fp.B[0], fp.Be[0], d, gc[0], gv[1], gv[2]);
+                B[1] = gather_cic(fp.B[1], fp.Be[1], d, gv[0], gc[1], gv[2]);
+                B[2] = gather_cic(fp.B[2], fp.Be[2], d, gv[0], gv[1], gc[2]);
+            } else {
+                GatherAxis gc[3], gv[3];
+                for (int a = 0; a < 3; ++a) {
+                    bool axact = d.n[a] > 1;
+                    gather_axis_fast(xf[a], ac[a].org_c, ac[a].isp_c, d.org_c[a], d.sp_c[a], sf, d.dims[a], axact,
+                                     d.g, gc[a], anc[a], dc[a]);
+                    gather_axis_fast(xf[a], ac[a].org_v, ac[a].isp_v, d.org_v[a], d.sp_v[a], sf, d.dims[a], axact,
+                                     d.g, gv[a], anv[a], dv[a]);
+                }
+                E[0] = gather3(fp.E[0], fp.Ee[0], d, gv[0], gc[1], gc[2]);
+                E[1] = gather3(fp.E[1], fp.Ee[1], d, gc[0], gv[1], gc[2]);
+                E[2] = gather3(fp.E[2], fp.Ee[2], d, gc[0], gc[1], gv[2]);
+                B[0] = gather3(fp.B[0], fp.Be[0], d, gc[0], gv[1], gv[2]);
+                B[1] = gather3(fp.B[1], fp.Be[1], d, gv[0], gc[1], gv[2]);
+                B[2] = gather3(fp.B[2], fp.Be[2], d, gv[0], gv[1], gc[2]);
+            }
             // ---- push (boris.py:14-124), masked by update_u (particle_push.py:134-142)
             float nv[3] = {v[0], v[1], v[2]};
             float h = (float)(sp.charge[s] * d.dt / (2.0 * sp.mass[s]));
@@ -159,6 +219,7 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
             double xn[3];
             bool regular = fast3d;
             float S0[3][2], D[3];
+#pragma unroll
             for (int a = 0; a < 3; ++a) {
                 bool ux = sp.update_x[s][a] != 0;
                 xn[a] = ux ? step_pos((double)xf[a], (double)v[a], d.dt) : (double)xf[a];
@@ -178,11 +239,14 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
                 // W = D_c * (S0 S0 + (S0 D + D S0)/2 + D D/3) over the other two axes,
                 // J_c = -(q/(d_o1 d_o2 dt)) * cumsum_c(W) -> value at slot a only.
                 double qw = sp.charge[s] * sp.weight[s];
+#pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     int o1 = c == 0 ? 1 : 0, o2 = c == 2 ? 1 : 2;
                     float pre = (float)(-(qw / (d.d[o1] * d.d[o2])) / d.dt) * (-D[c]);
+#pragma unroll
                     for (int j = 0; j < 2; ++j) {
                         float dj = j ? D[o1] : -D[o1];
+#pragma unroll
                         for (int k = 0; k < 2; ++k) {
                             float dk = k ? D[o2] : -D[o2];
                             float y = S0[o1][j] * S0[o2][k] + 0.5f * (S0[o1][j] * dk + dj * S0[o2][k]) +
@@ -227,6 +291,7 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
                 int so = __shfl_up(seg, dd);
                 bool same = lane >= dd && so == seg;
                 if (__ballot(same) == 0ull) break;
+#pragma unroll
                 for (int m = 0; m < 12; ++m) {
                     float o = __shfl_up(val[m], dd);
                     if (same) val[m] += o;
@@ -238,6 +303,7 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
                 const int64_t offx[4] = {0, 1, sy, sy + 1};       // (j,k) over (y,z)
                 const int64_t offy[4] = {0, 1, sx, sx + 1};       // (i,k) over (x,z)
                 const int64_t offz[4] = {0, sy, sx, sx + sy};     // (i,j) over (x,y)
+#pragma unroll
                 for (int m = 0; m < 4; ++m) {
                     if (val[m] != 0.0f) atomic_add_f32(&Jx[c0 + offx[m]], val[m]);
                     if (val[4 + m] != 0.0f) atomic_add_f32(&Jy[c0 + offy[m]], val[4 + m]);
@@ -253,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
     float* J[3] = {Jx, Jy, Jz};
     for (int q = threadIdx.x; q < nq; q += kThreads) {
         int64_t i = base + q_idx[q];
-        int s = (int)(i / sp.slots);
+        int s = species_of(i, sp.slots, sp.count);
         float v[3] = {q_v[0][q], q_v[1][q], q_v[2][q]};
         double xo[3], xn[3];
         load_pos(in, i, xo);
@@ -276,8 +342,15 @@ int pic_fused_push_esirkepov(const PicDesc* desc, const PicSpecies* sp, const Pi
     PIC_REQUIRE(field_size(*desc) < ((int64_t)1 << 31));
     int64_t n = (int64_t)sp->count * sp->slots;
     if (n == 0) return 0;
-    hipLaunchKernelGGL(fused_push_esirkepov_v2, dim3(grid_1d(n, kPerBlock)), dim3(kThreads), 0, as_stream(stream),
-                       *desc, *sp, *in, make_fields(E, B, E_ext, B_ext), *out, J[0], J[1], J[2], n);
+    bool fast = desc->shape_factor == 1 && desc->n[0] > 1 && desc->n[1] > 1 && desc->n[2] > 1;
+    if (fast)
+        hipLaunchKernelGGL(fused_push_esirkepov_v2<true>, dim3(grid_1d(n, kPerBlock)), dim3(kThreads), 0,
+                           as_stream(stream), *desc, *sp, *in, make_fields(E, B, E_ext, B_ext), *out, J[0], J[1], J[2],
+                           n);
+    else
+        hipLaunchKernelGGL(fused_push_esirkepov_v2<false>, dim3(grid_1d(n, kPerBlock)), dim3(kThreads), 0,
+                           as_stream(stream), *desc, *sp, *in, make_fields(E, B, E_ext, B_ext), *out, J[0], J[1], J[2],
+                           n);
     PIC_CHECK_LAUNCH();
     return 0;
 }

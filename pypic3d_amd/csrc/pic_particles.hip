@@ -25,7 +25,7 @@ __global__ void push_kernel(PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs
                             float* uz_o, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int s = (int)(i / sp.slots);
+    int s = species_of(i, sp.slots, sp.count);
     float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
     float nv[3] = {v[0], v[1], v[2]};
     if (in.active[i]) {
@@ -47,7 +47,7 @@ __global__ void advance_kernel(PicDesc d, PicSpecies sp, PicParticles in, double
                                int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int s = (int)(i / sp.slots);
+    int s = species_of(i, sp.slots, sp.count);
     bool act = in.active[i] != 0;
     float* out[3] = {xo, yo, zo};
     for (int a = 0; a < 3; ++a) {
@@ -80,7 +80,7 @@ __global__ void esirkepov_kernel(PicDesc d, PicSpecies sp, PicParticles in, floa
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (!in.active[i]) return;
-    int s = (int)(i / sp.slots);
+    int s = species_of(i, sp.slots, sp.count);
     double xo[3], xn[3];
     float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
     load_pos(in, i, xo);
@@ -93,7 +93,7 @@ __global__ void direct_kernel(PicDesc d, PicSpecies sp, PicParticles in, float* 
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (!in.active[i]) return;
-    int s = (int)(i / sp.slots);
+    int s = species_of(i, sp.slots, sp.count);
     double pos[3];
     load_pos(in, i, pos);
     NodeAxis ax[3];
@@ -115,7 +115,7 @@ __global__ void rho_kernel(PicDesc d, PicSpecies sp, PicParticles in, float* rho
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (!in.active[i]) return;
-    int s = (int)(i / sp.slots);
+    int s = species_of(i, sp.slots, sp.count);
     double pos[3];
     load_pos(in, i, pos);
     NodeAxis ax[3];

@@ -197,7 +197,7 @@ __global__ void energy_kinetic_kernel(PicSpecies sp, PicParticles in, double C, 
     double C2 = C * C;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (!in.active[i]) continue;
-        int s = (int)(i / sp.slots);
+        int s = species_of(i, sp.slots, sp.count);
         double m = sp.mass[s] * sp.weight[s];
         double vx = in.u[0][i], vy = in.u[1][i], vz = in.u[2][i];
         double v2 = vx * vx + vy * vy + vz * vz;

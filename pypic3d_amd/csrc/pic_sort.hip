@@ -20,7 +20,7 @@ constexpr int kBlock = 256;
 __global__ void cell_keys_kernel(PicDesc d, PicSpecies sp, PicParticles in, uint32_t* keys, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int s = (int)(i / sp.slots);
+    int s = species_of(i, sp.slots, sp.count);
     int64_t ncell = (int64_t)d.n[0] * d.n[1] * d.n[2];
     int64_t key = ncell;
     if (in.active[i]) {
