@@ -78,6 +78,7 @@ __device__ inline void gather_axis_fast(float x, float org32, float isp32, doubl
     float w[3];
     shape3f((float)dl * isp32, sf, w);
     if (active) {
+#pragma unroll
         for (int q = 0; q < 3; ++q) {
             s.p[q] = pmod(a - 1 + q, L);
             s.w[q] = w[q];
@@ -135,6 +136,70 @@ __device__ inline float gather_cic(const float* __restrict__ f, const float* __r
     return ax.w0 * (ay.w0 * c00 + ay.w1 * c01) + ax.w1 * (ay.w0 * c10 + ay.w1 * c11);
 }
 
+// First-order Esirkepov for a particle whose anchor moves by at most one cell
+// per axis, in a 3-slot frame starting at b = min(a_old, a_new): S0, D = S1 -
+// S0 (FP64) and the running sum C of D at slots 0 and 1 (slot 2's sum is 0).
+// The same values the 5-slot new-anchor frame of Esirkepov.py:158-189 holds
+// at those points; all other slots of that frame carry zero.
+struct CrossAxis {
+    int b;
+    float S0[3], D[3], C0, C1;
+};
+
+__device__ inline bool cross_axis(double xo, double xn, double org, double sp, double dd, CrossAxis& e) {
+    int ao = anchor_of(xo, org, sp, 1);
+    int an = anchor_of(xn, org, sp, 1);
+    if (an - ao > 1 || ao - an > 1) return false;
+    double ro = (xo - node_coord(ao, sp, org)) / dd;
+    double rn = (xn - node_coord(an, sp, org)) / dd;
+    int b = min(ao, an);
+    double s0[3] = {0.0, 0.0, 0.0}, s1[3] = {0.0, 0.0, 0.0};
+    if (ao == b) { s0[0] = 1.0 - ro; s0[1] = ro; } else { s0[1] = 1.0 - ro; s0[2] = ro; }
+    if (an == b) { s1[0] = 1.0 - rn; s1[1] = rn; } else { s1[1] = 1.0 - rn; s1[2] = rn; }
+    e.b = b;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        e.S0[k] = (float)s0[k];
+        e.D[k] = (float)(s1[k] - s0[k]);
+    }
+    e.C0 = (float)(s1[0] - s0[0]);
+    e.C1 = (float)((s1[0] - s0[0]) + (s1[1] - s0[1]));
+    return true;
+}
+
+__device__ inline float sel3(int k, float a0, float a1, float a2) { return k == 0 ? a0 : (k == 1 ? a1 : a2); }
+
+// Lane l < 54 deposits term (c, i, j, k): c = l / 18 the component, i in {0,1}
+// its differenced slot, (j, k) in 3x3 the two other axes in increasing order.
+__device__ inline void crossing_deposit(const PicDesc& d, double qw, CrossAxis c0, CrossAxis c1, CrossAxis c2,
+                                        int lane, float* Jx, float* Jy, float* Jz) {
+    if (lane >= 54) return;
+    int c = lane / 18, rem = lane % 18;
+    int i = rem / 9, jk = rem % 9;
+    int j = jk / 3, k = jk % 3;
+    // component axis A = c, other axes P = o1 < Q = o2; picked by value (selects, no dynamic indexing)
+    float ci = i == 0 ? sel3(c, c0.C0, c1.C0, c2.C0) : sel3(c, c0.C1, c1.C1, c2.C1);
+    // P is x unless c == 0 (then y); Q is z unless c == 2 (then y)
+    float ps0[3], pd[3], qs0[3], qd[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        ps0[t] = c == 0 ? c1.S0[t] : c0.S0[t];
+        pd[t] = c == 0 ? c1.D[t] : c0.D[t];
+        qs0[t] = c == 2 ? c1.S0[t] : c2.S0[t];
+        qd[t] = c == 2 ? c1.D[t] : c2.D[t];
+    }
+    float s0j = sel3(j, ps0[0], ps0[1], ps0[2]), dj = sel3(j, pd[0], pd[1], pd[2]);
+    float s0k = sel3(k, qs0[0], qs0[1], qs0[2]), dk = sel3(k, qd[0], qd[1], qd[2]);
+    float y = s0j * s0k + 0.5f * (s0j * dk + dj * s0k) + (1.0f / 3.0f) * dj * dk;
+    double dprod = c == 0 ? d.d[1] * d.d[2] : (c == 1 ? d.d[0] * d.d[2] : d.d[0] * d.d[1]);
+    float val = (float)(-(qw / dprod) / d.dt) * ci * y;
+    int ix = c0.b + (c == 0 ? i : j);
+    int iy = c1.b + (c == 0 ? j : (c == 1 ? i : k));
+    int iz = c2.b + (c == 2 ? i : k);
+    float* J = c == 0 ? Jx : (c == 1 ? Jy : Jz);
+    deposit_at(J, d, ix, iy, iz, val);
+}
+
 template <bool kFast>
 __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, PicSpecies sp, PicParticles in,
                                                                     FieldPtrs fp, PicParticles out, float* Jx,
@@ -149,6 +214,7 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
     const int sf = d.shape_factor;
     const bool fast3d = kFast;
     AxisConst ac[3];
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
         ac[a].org_c = (float)d.org_c[a];
         ac[a].org_v = (float)d.org_v[a];
@@ -195,6 +261,7 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
                 B[2] = gather_cic(fp.B[2], fp.Be[2], d, gv[0], gv[1], gc[2]);
             } else {
                 GatherAxis gc[3], gv[3];
+#pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     bool axact = d.n[a] > 1;
                     gather_axis_fast(xf[a], ac[a].org_c, ac[a].isp_c, d.org_c[a], d.sp_c[a], sf, d.dims[a], axact,
@@ -213,6 +280,7 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
             float nv[3] = {v[0], v[1], v[2]};
             float h = (float)(sp.charge[s] * d.dt / (2.0 * sp.mass[s]));
             boris_push(nv, E, B, h, d.relativistic, (float)d.C);
+#pragma unroll
             for (int a = 0; a < 3; ++a)
                 if (sp.update_u[s][a]) v[a] = nv[a];
             // ---- new position (Esirkepov.py:125-127) and regular-case test
@@ -265,13 +333,16 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
                 q_v[2][slot] = v[2];
             }
             // ---- position update + particle boundaries (particle_tile_communication.py)
+#pragma unroll
             for (int a = 0; a < 3; ++a) apply_axis_boundary(d.pbc[a], d.wind[a], xn[a], v[a], act);
+#pragma unroll
             for (int a = 0; a < 3; ++a) {
                 out.x[a][i] = act ? (float)xn[a] : 0.0f;
                 out.u[a][i] = act ? v[a] : 0.0f;
             }
             out.active[i] = act ? 1 : 0;
         } else if (valid) {
+#pragma unroll
             for (int a = 0; a < 3; ++a) {
                 out.x[a][i] = 0.0f;
                 out.u[a][i] = 0.0f;
@@ -314,17 +385,41 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
     }
     __syncthreads();
 
-    // ---- general path for queued particles (Esirkepov.py:105-331)
+    // ---- queued particles (Esirkepov.py:105-331)
     const int nq = q_n;
     float* J[3] = {Jx, Jy, Jz};
-    for (int q = threadIdx.x; q < nq; q += kThreads) {
-        int64_t i = base + q_idx[q];
-        int s = species_of(i, sp.slots, sp.count);
-        float v[3] = {q_v[0][q], q_v[1][q], q_v[2][q]};
-        double xo[3], xn[3];
-        load_pos(in, i, xo);
-        for (int a = 0; a < 3; ++a) xn[a] = sp.update_x[s][a] ? step_pos(xo[a], (double)v[a], d.dt) : xo[a];
-        esirkepov_particle(d, sp.charge[s] * sp.weight[s], xo, xn, v, J);
+    if constexpr (kFast) {
+        // one wave per queued particle; lane l owns one (component, i, j, k) term
+        const int wave = threadIdx.x >> 6;
+        for (int q = wave; q < nq; q += kThreads / 64) {
+            int64_t i = base + q_idx[q];
+            int s = species_of(i, sp.slots, sp.count);
+            float v[3] = {q_v[0][q], q_v[1][q], q_v[2][q]};
+            double xo[3], xn[3];
+            load_pos(in, i, xo);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) xn[a] = sp.update_x[s][a] ? step_pos(xo[a], (double)v[a], d.dt) : xo[a];
+            CrossAxis cx, cy, cz;
+            bool ok = cross_axis(xo[0], xn[0], d.org_c[0], d.sp_c[0], d.d[0], cx);
+            ok = cross_axis(xo[1], xn[1], d.org_c[1], d.sp_c[1], d.d[1], cy) && ok;
+            ok = cross_axis(xo[2], xn[2], d.org_c[2], d.sp_c[2], d.d[2], cz) && ok;
+            if (!ok) {   // moved two or more cells: the general 5-slot path
+                if (lane == 0) esirkepov_particle(d, sp.charge[s] * sp.weight[s], xo, xn, v, J);
+                continue;
+            }
+            crossing_deposit(d, sp.charge[s] * sp.weight[s], cx, cy, cz, lane, Jx, Jy, Jz);
+        }
+    } else {
+        for (int q = threadIdx.x; q < nq; q += kThreads) {
+            int64_t i = base + q_idx[q];
+            int s = species_of(i, sp.slots, sp.count);
+            float v[3] = {q_v[0][q], q_v[1][q], q_v[2][q]};
+            double xo[3], xn[3];
+            load_pos(in, i, xo);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) xn[a] = sp.update_x[s][a] ? step_pos(xo[a], (double)v[a], d.dt) : xo[a];
+            esirkepov_particle(d, sp.charge[s] * sp.weight[s], xo, xn, v, J);
+        }
     }
 }
 
@@ -337,6 +432,7 @@ int pic_fused_push_esirkepov(const PicDesc* desc, const PicSpecies* sp, const Pi
                              const float* const B_ext[3], PicParticles* out, float* const J[3], void* stream) {
     PIC_REQUIRE(desc_ok(desc) && particles_ok(sp, in) && particles_ok(sp, out) && E && B && J);
     PIC_REQUIRE(desc->n[0] > 1 || desc->n[1] > 1 || desc->n[2] > 1);
+#pragma unroll
     for (int a = 0; a < 3; ++a) PIC_REQUIRE(E[a] && B[a] && J[a] && in->x[a] != out->x[a] && in->u[a] != out->u[a]);
     PIC_REQUIRE(in->active != out->active);
     PIC_REQUIRE(field_size(*desc) < ((int64_t)1 << 31));
