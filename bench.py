@@ -60,7 +60,20 @@ def main():
     ap.add_argument("--ppc", type=int, default=32)
     ap.add_argument("--sort-every", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
+                    help="only run a plain device copy of MiB MiB five times (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": 4 * n, "calibrate_copy_bytes_written": 4 * n, "launches": 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU (domain-decomposed) path is "
                                    "design-only, so --gpus must be 1"}))
@@ -83,6 +96,10 @@ def main():
 
     for _ in range(args.warmup):
         runner.step()
+    torch.cuda.synchronize()
+    variant, tune_ms = runner.autotune() if not args.no_autotune else (0, {})
+    print(f"[bench] autotune fused variants (ms): {tune_ms} -> {variant}", file=sys.stderr, flush=True)
+    runner.step()
     torch.cuda.synchronize()
 
     runner.enable_timing(True)
@@ -121,6 +138,7 @@ def main():
                         "CIC Esirkepov, relativistic Boris, Yee",
             "particles": n_particles, "cells": n_cells, "sort_every": args.sort_every,
             "resorted_particles_per_step": n_particles * sorts / args.steps,
+            "fused_variant": variant, "autotune_ms": tune_ms,
         },
         "roofline": {
             "bound": "hbm", "kernel": "fused_push_esirkepov_kernel",

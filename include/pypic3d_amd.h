@@ -41,7 +41,7 @@ typedef struct PicDesc {
     int32_t relativistic;  /* relativistic Boris (boris.py:60-124)            */
     int32_t bc[3];         /* field BC: 0 periodic, 1 conducting              */
     int32_t pbc[3];        /* particle BC: 0 periodic, 1 reflect, 2 absorb    */
-    int32_t pad0;
+    int32_t variant;       /* kernel variant selector (tuning); 0 = default   */
     double d[3];           /* dx, dy, dz                                      */
     double wind[3];        /* x_wind, y_wind, z_wind                          */
     double org_c[3];       /* tiled "center" line: first coordinate           */
@@ -127,7 +127,10 @@ int pic_deposit_rho(const PicDesc* desc, const PicSpecies* sp, const PicParticle
                     float* rho, void* stream);
 /* Fused particle phase of time_loop_electrodynamic's Esirkepov branch
  * (evolve.py:36-43, 70-79): gather + push, raw Esirkepov deposit into J (zeroed
- * by the caller), x += v dt, particle boundaries.  Writes out (may alias in). */
+ * by the caller), x += v dt, particle boundaries.  Writes out (must not alias
+ * in).  desc->variant picks one of PIC_FUSED_VARIANTS launch shapes (slots per
+ * thread x resident-block target) for the 3-D first-order case. */
+#define PIC_FUSED_VARIANTS 6
 int pic_fused_push_esirkepov(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
                              const float* const E[3], const float* const B[3],
                              const float* const E_ext[3], const float* const B_ext[3],

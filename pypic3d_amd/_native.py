@@ -15,6 +15,7 @@ import torch
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpypic3d_hip.so")
 MAX_SPECIES = 16
+FUSED_VARIANTS = 6
 ABI_VERSION = 1
 
 c_int = ctypes.c_int
@@ -28,7 +29,7 @@ P3 = c_vp * 3
 class PicDesc(ctypes.Structure):
     _fields_ = [
         ("n", c_i32 * 3), ("g", c_i32), ("dims", c_i32 * 3), ("shape_factor", c_i32),
-        ("relativistic", c_i32), ("bc", c_i32 * 3), ("pbc", c_i32 * 3), ("pad0", c_i32),
+        ("relativistic", c_i32), ("bc", c_i32 * 3), ("pbc", c_i32 * 3), ("variant", c_i32),
         ("d", c_dbl * 3), ("wind", c_dbl * 3), ("org_c", c_dbl * 3), ("sp_c", c_dbl * 3),
         ("org_v", c_dbl * 3), ("sp_v", c_dbl * 3),
         ("dt", c_dbl), ("C", c_dbl), ("eps", c_dbl), ("mu", c_dbl), ("alpha", c_dbl),

@@ -27,8 +27,6 @@ namespace {
 using namespace pic;
 
 constexpr int kThreads = 256;
-constexpr int kRounds = 4;
-constexpr int kPerBlock = kThreads * kRounds;
 
 // Anchor from an FP32 estimate t of (x - origin)/spacing: taken as is when it
 // sits farther than `margin` from a face (floor) or a half-cell (round, TSC);
@@ -116,20 +114,22 @@ __device__ inline void cic_axis(float x, float org32, float isp32, double org, d
 
 __device__ inline float gather_cic(const float* __restrict__ f, const float* __restrict__ fe, const PicDesc& d,
                                    const CicAxis& ax, const CicAxis& ay, const CicAxis& az) {
-    const int64_t sy = d.dims[2], sx = (int64_t)d.dims[1] * d.dims[2];
-    const int64_t r00 = ax.p0 * sx + ay.p0 * sy, r01 = ax.p0 * sx + ay.p1 * sy;
-    const int64_t r10 = ax.p1 * sx + ay.p0 * sy, r11 = ax.p1 * sx + ay.p1 * sy;
-    float v000 = f[r00 + az.p0], v001 = f[r00 + az.p1], v010 = f[r01 + az.p0], v011 = f[r01 + az.p1];
-    float v100 = f[r10 + az.p0], v101 = f[r10 + az.p1], v110 = f[r11 + az.p0], v111 = f[r11 + az.p1];
+    // 32-bit unsigned offsets on a uniform base (field_size < 2^31 is checked at launch)
+    const uint32_t sy = (uint32_t)d.dims[2], sx = (uint32_t)d.dims[1] * sy;
+    const uint32_t r00 = (uint32_t)ax.p0 * sx + (uint32_t)ay.p0 * sy, r01 = (uint32_t)ax.p0 * sx + (uint32_t)ay.p1 * sy;
+    const uint32_t r10 = (uint32_t)ax.p1 * sx + (uint32_t)ay.p0 * sy, r11 = (uint32_t)ax.p1 * sx + (uint32_t)ay.p1 * sy;
+    const uint32_t z0 = (uint32_t)az.p0, z1 = (uint32_t)az.p1;
+    float v000 = f[r00 + z0], v001 = f[r00 + z1], v010 = f[r01 + z0], v011 = f[r01 + z1];
+    float v100 = f[r10 + z0], v101 = f[r10 + z1], v110 = f[r11 + z0], v111 = f[r11 + z1];
     if (fe) {
-        v000 += fe[r00 + az.p0];
-        v001 += fe[r00 + az.p1];
-        v010 += fe[r01 + az.p0];
-        v011 += fe[r01 + az.p1];
-        v100 += fe[r10 + az.p0];
-        v101 += fe[r10 + az.p1];
-        v110 += fe[r11 + az.p0];
-        v111 += fe[r11 + az.p1];
+        v000 += fe[r00 + z0];
+        v001 += fe[r00 + z1];
+        v010 += fe[r01 + z0];
+        v011 += fe[r01 + z1];
+        v100 += fe[r10 + z0];
+        v101 += fe[r10 + z1];
+        v110 += fe[r11 + z0];
+        v111 += fe[r11 + z1];
     }
     float c00 = v000 * az.w0 + v001 * az.w1, c01 = v010 * az.w0 + v011 * az.w1;
     float c10 = v100 * az.w0 + v101 * az.w1, c11 = v110 * az.w0 + v111 * az.w1;
@@ -146,12 +146,14 @@ struct CrossAxis {
     float S0[3], D[3], C0, C1;
 };
 
-__device__ inline bool cross_axis(double xo, double xn, double org, double sp, double dd, CrossAxis& e) {
-    int ao = anchor_of(xo, org, sp, 1);
-    int an = anchor_of(xn, org, sp, 1);
+__device__ inline bool cross_axis(double xo, double xn, double org, double sp, double inv_d, float org32,
+                                  float isp32, CrossAxis& e) {
+    float mag = (fabsf((float)xo) + fabsf(org32) + 1.0f) * isp32;
+    int ao = anchor_est(((float)xo - org32) * isp32, mag, xo, org, sp, 1);
+    int an = anchor_est(((float)xn - org32) * isp32, mag + fabsf(((float)xn - (float)xo) * isp32), xn, org, sp, 1);
     if (an - ao > 1 || ao - an > 1) return false;
-    double ro = (xo - node_coord(ao, sp, org)) / dd;
-    double rn = (xn - node_coord(an, sp, org)) / dd;
+    double ro = (xo - node_coord(ao, sp, org)) * inv_d;
+    double rn = (xn - node_coord(an, sp, org)) * inv_d;
     int b = min(ao, an);
     double s0[3] = {0.0, 0.0, 0.0}, s1[3] = {0.0, 0.0, 0.0};
     if (ao == b) { s0[0] = 1.0 - ro; s0[1] = ro; } else { s0[1] = 1.0 - ro; s0[2] = ro; }
@@ -169,41 +171,48 @@ __device__ inline bool cross_axis(double xo, double xn, double org, double sp, d
 
 __device__ inline float sel3(int k, float a0, float a1, float a2) { return k == 0 ? a0 : (k == 1 ? a1 : a2); }
 
-// Lane l < 54 deposits term (c, i, j, k): c = l / 18 the component, i in {0,1}
-// its differenced slot, (j, k) in 3x3 the two other axes in increasing order.
-__device__ inline void crossing_deposit(const PicDesc& d, double qw, CrossAxis c0, CrossAxis c1, CrossAxis c2,
-                                        int lane, float* Jx, float* Jy, float* Jz) {
-    if (lane >= 54) return;
-    int c = lane / 18, rem = lane % 18;
-    int i = rem / 9, jk = rem % 9;
-    int j = jk / 3, k = jk % 3;
-    // component axis A = c, other axes P = o1 < Q = o2; picked by value (selects, no dynamic indexing)
-    float ci = i == 0 ? sel3(c, c0.C0, c1.C0, c2.C0) : sel3(c, c0.C1, c1.C1, c2.C1);
-    // P is x unless c == 0 (then y); Q is z unless c == 2 (then y)
-    float ps0[3], pd[3], qs0[3], qd[3];
+// All 54 terms (component c, differenced slot i in {0,1}, 3x3 slots (j, k) of
+// the other two axes) of one particle, fully unrolled so every per-axis value
+// stays in registers.
+template <int kC>
+__device__ inline void crossing_component(const PicDesc& d, double qw, const CrossAxis& A, const CrossAxis& P,
+                                          const CrossAxis& Q, float* J) {
+    constexpr int o1 = kC == 0 ? 1 : 0, o2 = kC == 2 ? 1 : 2;
+    const float pre = (float)(-(qw / (d.d[o1] * d.d[o2])) / d.dt);
 #pragma unroll
-    for (int t = 0; t < 3; ++t) {
-        ps0[t] = c == 0 ? c1.S0[t] : c0.S0[t];
-        pd[t] = c == 0 ? c1.D[t] : c0.D[t];
-        qs0[t] = c == 2 ? c1.S0[t] : c2.S0[t];
-        qd[t] = c == 2 ? c1.D[t] : c2.D[t];
+    for (int i = 0; i < 2; ++i) {
+        const float ci = i == 0 ? A.C0 : A.C1;
+        if (ci == 0.0f) continue;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                float y = P.S0[j] * Q.S0[k] + 0.5f * (P.S0[j] * Q.D[k] + P.D[j] * Q.S0[k]) +
+                          (1.0f / 3.0f) * P.D[j] * Q.D[k];
+                int ic = A.b + i, i1 = P.b + j, i2 = Q.b + k;
+                int ix = kC == 0 ? ic : i1;
+                int iy = kC == 0 ? i1 : (kC == 1 ? ic : i2);
+                int iz = kC == 2 ? ic : i2;
+                deposit_at(J, d, ix, iy, iz, pre * ci * y);
+            }
+        }
     }
-    float s0j = sel3(j, ps0[0], ps0[1], ps0[2]), dj = sel3(j, pd[0], pd[1], pd[2]);
-    float s0k = sel3(k, qs0[0], qs0[1], qs0[2]), dk = sel3(k, qd[0], qd[1], qd[2]);
-    float y = s0j * s0k + 0.5f * (s0j * dk + dj * s0k) + (1.0f / 3.0f) * dj * dk;
-    double dprod = c == 0 ? d.d[1] * d.d[2] : (c == 1 ? d.d[0] * d.d[2] : d.d[0] * d.d[1]);
-    float val = (float)(-(qw / dprod) / d.dt) * ci * y;
-    int ix = c0.b + (c == 0 ? i : j);
-    int iy = c1.b + (c == 0 ? j : (c == 1 ? i : k));
-    int iz = c2.b + (c == 2 ? i : k);
-    float* J = c == 0 ? Jx : (c == 1 ? Jy : Jz);
-    deposit_at(J, d, ix, iy, iz, val);
 }
 
-template <bool kFast>
-__global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, PicSpecies sp, PicParticles in,
+__device__ inline void crossing_deposit(const PicDesc& d, double qw, const CrossAxis& cx, const CrossAxis& cy,
+                                        const CrossAxis& cz, float* Jx, float* Jy, float* Jz) {
+    crossing_component<0>(d, qw, cx, cy, cz, Jx);
+    crossing_component<1>(d, qw, cy, cx, cz, Jy);
+    crossing_component<2>(d, qw, cz, cx, cy, Jz);
+}
+
+// kRounds: particle slots per thread (block = kThreads * kRounds slots);
+// kMinBlocks: resident-block target handed to the register allocator.
+template <bool kFast, int kRounds, int kMinBlocks>
+__global__ __launch_bounds__(kThreads, kMinBlocks) void fused_push_esirkepov_v2(PicDesc d, PicSpecies sp, PicParticles in,
                                                                     FieldPtrs fp, PicParticles out, float* Jx,
                                                                     float* Jy, float* Jz, int64_t n) {
+    constexpr int kPerBlock = kThreads * kRounds;
     __shared__ int q_idx[kPerBlock];
     __shared__ float q_v[3][kPerBlock];
     __shared__ int q_n;
@@ -222,14 +231,14 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
         ac[a].isp_v = (float)(1.0 / d.sp_v[a]);
         ac[a].inv_d = (float)(1.0 / d.d[a]);
     }
-    const int64_t sx = (int64_t)d.dims[1] * d.dims[2];
-    const int64_t sy = d.dims[2];
+    const uint32_t sy = (uint32_t)d.dims[2];
+    const uint32_t sx = (uint32_t)d.dims[1] * sy;
     const float fdt = (float)d.dt;
-    const int64_t base = (int64_t)blockIdx.x * kPerBlock;
+    const uint32_t base = blockIdx.x * (uint32_t)kPerBlock;   // n < 2^31 is checked at launch
 
     for (int r = 0; r < kRounds; ++r) {
         const int local = r * kThreads + threadIdx.x;
-        const int64_t i = base + local;
+        const uint32_t i = base + (uint32_t)local;
         const bool valid = i < n;
         const int s = valid ? species_of(i, sp.slots, sp.count) : 0;
         bool act = valid && in.active[i] != 0;
@@ -370,10 +379,10 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
             }
             bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
             if (tail && key >= 0) {
-                const int64_t c0 = cell;
-                const int64_t offx[4] = {0, 1, sy, sy + 1};       // (j,k) over (y,z)
-                const int64_t offy[4] = {0, 1, sx, sx + 1};       // (i,k) over (x,z)
-                const int64_t offz[4] = {0, sy, sx, sx + sy};     // (i,j) over (x,y)
+                const uint32_t c0 = (uint32_t)cell;
+                const uint32_t offx[4] = {0u, 1u, sy, sy + 1u};       // (j,k) over (y,z)
+                const uint32_t offy[4] = {0u, 1u, sx, sx + 1u};       // (i,k) over (x,z)
+                const uint32_t offz[4] = {0u, sy, sx, sx + sy};       // (i,j) over (x,y)
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
                     if (val[m] != 0.0f) atomic_add_f32(&Jx[c0 + offx[m]], val[m]);
@@ -389,10 +398,9 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
     const int nq = q_n;
     float* J[3] = {Jx, Jy, Jz};
     if constexpr (kFast) {
-        // one wave per queued particle; lane l owns one (component, i, j, k) term
-        const int wave = threadIdx.x >> 6;
-        for (int q = wave; q < nq; q += kThreads / 64) {
-            int64_t i = base + q_idx[q];
+        // one lane per queued particle, all terms unrolled (no scratch arrays)
+        for (int q = threadIdx.x; q < nq; q += kThreads) {
+            uint32_t i = base + (uint32_t)q_idx[q];
             int s = species_of(i, sp.slots, sp.count);
             float v[3] = {q_v[0][q], q_v[1][q], q_v[2][q]};
             double xo[3], xn[3];
@@ -400,18 +408,16 @@ __global__ __launch_bounds__(kThreads) void fused_push_esirkepov_v2(PicDesc d, P
 #pragma unroll
             for (int a = 0; a < 3; ++a) xn[a] = sp.update_x[s][a] ? step_pos(xo[a], (double)v[a], d.dt) : xo[a];
             CrossAxis cx, cy, cz;
-            bool ok = cross_axis(xo[0], xn[0], d.org_c[0], d.sp_c[0], d.d[0], cx);
-            ok = cross_axis(xo[1], xn[1], d.org_c[1], d.sp_c[1], d.d[1], cy) && ok;
-            ok = cross_axis(xo[2], xn[2], d.org_c[2], d.sp_c[2], d.d[2], cz) && ok;
-            if (!ok) {   // moved two or more cells: the general 5-slot path
-                if (lane == 0) esirkepov_particle(d, sp.charge[s] * sp.weight[s], xo, xn, v, J);
-                continue;
-            }
-            crossing_deposit(d, sp.charge[s] * sp.weight[s], cx, cy, cz, lane, Jx, Jy, Jz);
+            bool ok = cross_axis(xo[0], xn[0], d.org_c[0], d.sp_c[0], 1.0 / d.d[0], ac[0].org_c, ac[0].isp_c, cx);
+            ok = cross_axis(xo[1], xn[1], d.org_c[1], d.sp_c[1], 1.0 / d.d[1], ac[1].org_c, ac[1].isp_c, cy) && ok;
+            ok = cross_axis(xo[2], xn[2], d.org_c[2], d.sp_c[2], 1.0 / d.d[2], ac[2].org_c, ac[2].isp_c, cz) && ok;
+            double qw = sp.charge[s] * sp.weight[s];
+            if (ok) crossing_deposit(d, qw, cx, cy, cz, Jx, Jy, Jz);
+            else esirkepov_particle(d, qw, xo, xn, v, J);   // moved two or more cells
         }
     } else {
         for (int q = threadIdx.x; q < nq; q += kThreads) {
-            int64_t i = base + q_idx[q];
+            uint32_t i = base + (uint32_t)q_idx[q];
             int s = species_of(i, sp.slots, sp.count);
             float v[3] = {q_v[0][q], q_v[1][q], q_v[2][q]};
             double xo[3], xn[3];
@@ -437,16 +443,27 @@ int pic_fused_push_esirkepov(const PicDesc* desc, const PicSpecies* sp, const Pi
     PIC_REQUIRE(in->active != out->active);
     PIC_REQUIRE(field_size(*desc) < ((int64_t)1 << 31));
     int64_t n = (int64_t)sp->count * sp->slots;
+    PIC_REQUIRE(n < ((int64_t)1 << 31) - 8 * kThreads);
     if (n == 0) return 0;
     bool fast = desc->shape_factor == 1 && desc->n[0] > 1 && desc->n[1] > 1 && desc->n[2] > 1;
-    if (fast)
-        hipLaunchKernelGGL(fused_push_esirkepov_v2<true>, dim3(grid_1d(n, kPerBlock)), dim3(kThreads), 0,
-                           as_stream(stream), *desc, *sp, *in, make_fields(E, B, E_ext, B_ext), *out, J[0], J[1], J[2],
-                           n);
-    else
-        hipLaunchKernelGGL(fused_push_esirkepov_v2<false>, dim3(grid_1d(n, kPerBlock)), dim3(kThreads), 0,
-                           as_stream(stream), *desc, *sp, *in, make_fields(E, B, E_ext, B_ext), *out, J[0], J[1], J[2],
-                           n);
+    hipStream_t st = as_stream(stream);
+    FieldPtrs fp = make_fields(E, B, E_ext, B_ext);
+#define PIC_FUSED_LAUNCH(FAST, R, MB)                                                                           \
+    hipLaunchKernelGGL((fused_push_esirkepov_v2<FAST, R, MB>), dim3(grid_1d(n, kThreads * R)), dim3(kThreads), 0, \
+                       st, *desc, *sp, *in, fp, *out, J[0], J[1], J[2], n)
+    if (!fast) {
+        PIC_FUSED_LAUNCH(false, 4, 1);
+    } else {
+        switch (desc->variant) {
+            case 1: PIC_FUSED_LAUNCH(true, 4, 3); break;
+            case 2: PIC_FUSED_LAUNCH(true, 4, 4); break;
+            case 3: PIC_FUSED_LAUNCH(true, 8, 1); break;
+            case 4: PIC_FUSED_LAUNCH(true, 2, 1); break;
+            case 5: PIC_FUSED_LAUNCH(true, 8, 3); break;
+            default: PIC_FUSED_LAUNCH(true, 4, 1); break;
+        }
+    }
+#undef PIC_FUSED_LAUNCH
     PIC_CHECK_LAUNCH();
     return 0;
 }

@@ -90,16 +90,20 @@ __device__ inline void cross(const float a[3], const float b[3], float o[3]) {
     o[2] = a[0] * b[1] - a[1] * b[0];
 }
 
+// Boris rotation with hardware reciprocal / reciprocal-square-root (1 ulp),
+// the same algebra as boris.py:14-124: gamma = 1/sqrt(1 - v^2/C^2),
+// u- = gamma v + hE, t = hB / sqrt(1 + u-^2/C^2), u' = u- + u- x t,
+// s = 2t / (1 + t^2), u+ = u- + u' x s, u = u+ + hE, v = u / sqrt(1 + u^2/C^2).
 __device__ inline void boris_push(float v[3], const float E[3], const float B[3], float h, int relativistic,
                                   float C) {
     float um[3], t[3], up[3], s[3], c1[3];
+    const float invC2 = __builtin_amdgcn_rcpf(C * C);
     if (relativistic) {
         float v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-        float invC2 = 1.0f / (C * C);
-        float gamma = 1.0f / sqrtf(1.0f - v2 * invC2);
+        float gamma = __builtin_amdgcn_rsqf(1.0f - v2 * invC2);
         for (int a = 0; a < 3; ++a) um[a] = v[a] * gamma + h * E[a];
-        float gm = sqrtf(1.0f + (um[0] * um[0] + um[1] * um[1] + um[2] * um[2]) * invC2);
-        for (int a = 0; a < 3; ++a) t[a] = h * B[a] / gm;
+        float igm = __builtin_amdgcn_rsqf(1.0f + (um[0] * um[0] + um[1] * um[1] + um[2] * um[2]) * invC2);
+        for (int a = 0; a < 3; ++a) t[a] = h * B[a] * igm;
     } else {
         for (int a = 0; a < 3; ++a) {
             um[a] = v[a] + h * E[a];
@@ -108,14 +112,14 @@ __device__ inline void boris_push(float v[3], const float E[3], const float B[3]
     }
     cross(um, t, c1);
     for (int a = 0; a < 3; ++a) up[a] = um[a] + c1[a];
-    float den = 1.0f + t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
-    for (int a = 0; a < 3; ++a) s[a] = 2.0f * t[a] / den;
+    float f = 2.0f * __builtin_amdgcn_rcpf(1.0f + t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    for (int a = 0; a < 3; ++a) s[a] = t[a] * f;
     cross(up, s, c1);
     float nu[3];
     for (int a = 0; a < 3; ++a) nu[a] = um[a] + c1[a] + h * E[a];
     if (relativistic) {
-        float ng = sqrtf(1.0f + (nu[0] * nu[0] + nu[1] * nu[1] + nu[2] * nu[2]) / (C * C));
-        for (int a = 0; a < 3; ++a) v[a] = nu[a] / ng;
+        float ing = __builtin_amdgcn_rsqf(1.0f + (nu[0] * nu[0] + nu[1] * nu[1] + nu[2] * nu[2]) * invC2);
+        for (int a = 0; a < 3; ++a) v[a] = nu[a] * ing;
     } else {
         for (int a = 0; a < 3; ++a) v[a] = nu[a];
     }

@@ -142,6 +142,20 @@ class ElectrodynamicRunner:
         if self.sort_every and self.steps_done % self.sort_every == 0:
             self.sort()
 
+    def autotune(self, reps=3):
+        """Pick the fastest fused-kernel launch shape on the current state.
+        The timed launches write the inactive particle set and J, which the
+        next step overwrites."""
+        from .autotune import tune_fused
+        if not self.esirkepov:
+            return None, {}
+        src, dst = self.cur, 1 - self.cur
+        E = self.E[self.ecur]
+        best, times = tune_fused(self.desc, self.sp, self._pp(src), E, self.B, self.Ee, self.Be, self._pp(dst),
+                                 self.J, reps=reps)
+        self.desc.variant = best
+        return best, times
+
     def sort(self):
         """Stable cell sort of the current particle buffers (into the other set)."""
         s = N.stream_handle()
