@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define PIC_ABI_VERSION 1
+#define PIC_ABI_VERSION 2
 #define PIC_MAX_SPECIES 16
 
 /* Scalar run description: StaticParameters + DynamicParameters fields the hot
@@ -49,6 +49,8 @@ typedef struct PicDesc {
     double org_v[3];       /* tiled "vertex" line                             */
     double sp_v[3];
     double dt, C, eps, mu, alpha;
+    int32_t pusher;        /* 0 Boris (relativistic per the flag above),      */
+    int32_t pad1;          /* 1 Higuera-Cary (higuera_cary.py:57-113)         */
 } PicDesc;
 
 /* SpeciesConfig (particles/particle_class.py:5-14). */
@@ -101,7 +103,8 @@ int pic_yee_update_e(const PicDesc* desc, const float* const E_in[3], const floa
 
 /* ---- particles --------------------------------------------------------- */
 /* particle_push (pusher/particle_push.py:13-175): gather E+E_ext, B+B_ext on
- * the Yee-staggered grids, Boris or relativistic Boris; writes u_out (may
+ * the Yee-staggered grids, Boris / relativistic Boris / Higuera-Cary per
+ * desc->pusher and desc->relativistic (particle_push.py:109-132); writes u_out (may
  * alias u).  E_ext/B_ext may be NULL (zero external field). */
 int pic_push(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
              const float* const E[3], const float* const B[3],

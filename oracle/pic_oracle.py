@@ -387,6 +387,25 @@ def relativistic_boris(v, E, B, q, m, dt, C):
     return newu / new_gamma
 
 
+def higuera_cary(v, E, B, q, m, dt, C):
+    """higuera_cary_single_particle (higuera_cary.py:5-113), vectorised; (3, N)."""
+    u = v / np.sqrt(1.0 - (v[0] ** 2 + v[1] ** 2 + v[2] ** 2) / C ** 2)
+    h = q * dt / (2.0 * m)
+    eps = h * E
+    beta = h * B
+    ue = u + eps
+    beta2 = beta[0] ** 2 + beta[1] ** 2 + beta[2] ** 2
+    ustar = (ue[0] * beta[0] + ue[1] * beta[1] + ue[2] * beta[2]) / C
+    sigma = (1.0 + (ue[0] ** 2 + ue[1] ** 2 + ue[2] ** 2) / C ** 2) - beta2
+    gamma = np.sqrt((sigma + np.sqrt(sigma ** 2 + 4.0 * (beta2 + ustar ** 2))) / 2.0)
+    t = beta / gamma
+    s = 1.0 / (1.0 + t[0] ** 2 + t[1] ** 2 + t[2] ** 2)
+    uet = ue[0] * t[0] + ue[1] * t[1] + ue[2] * t[2]
+    um = s * (ue + uet * t + np.cross(ue, t, axis=0))
+    newu = um + eps + np.cross(um, t, axis=0)
+    return newu / np.sqrt(1.0 + (newu[0] ** 2 + newu[1] ** 2 + newu[2] ** 2) / C ** 2)
+
+
 @dataclass
 class Species:
     """SpeciesConfig (particle_class.py:5-14)."""
@@ -399,7 +418,7 @@ class Species:
 
 def particle_push(x, u, active, sp: Species, E, B, p: Params):
     """particle_push (particle_push.py:13-175) for one tile; returns new u."""
-    if p.particle_pusher != "boris":
+    if p.particle_pusher not in ("boris", "higuera_cary"):
         raise ValueError(f"Unknown particle_pusher: {p.particle_pusher}")
     S, P, _ = x.shape
     lines = component_lines(p)
@@ -413,7 +432,9 @@ def particle_push(x, u, active, sp: Species, E, B, p: Params):
     v = np.stack([u[..., a].reshape(-1) for a in range(3)])
     q = np.repeat(np.asarray(sp.charge, dtype=np.float64), P)
     m = np.repeat(np.asarray(sp.mass, dtype=np.float64), P)
-    if p.relativistic:
+    if p.particle_pusher == "higuera_cary":        # particle_push.py:124-131
+        newv = higuera_cary(v, Ef, Bf, q, m, p.dt, p.C)
+    elif p.relativistic:
         newv = relativistic_boris(v, Ef, Bf, q, m, p.dt, p.C)
     else:
         newv = boris(v, Ef, Bf, q, m, p.dt)

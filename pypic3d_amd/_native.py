@@ -15,8 +15,9 @@ import torch
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpypic3d_hip.so")
 MAX_SPECIES = 16
+PUSHERS = {"boris": 0, "higuera_cary": 1}
 FUSED_VARIANTS = 6
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 c_int = ctypes.c_int
 c_i32 = ctypes.c_int32
@@ -33,6 +34,7 @@ class PicDesc(ctypes.Structure):
         ("d", c_dbl * 3), ("wind", c_dbl * 3), ("org_c", c_dbl * 3), ("sp_c", c_dbl * 3),
         ("org_v", c_dbl * 3), ("sp_v", c_dbl * 3),
         ("dt", c_dbl), ("C", c_dbl), ("eps", c_dbl), ("mu", c_dbl), ("alpha", c_dbl),
+        ("pusher", c_i32), ("pad1", c_i32),
     ]
 
 

@@ -52,6 +52,9 @@ def make_desc(static, dynamic) -> N.PicDesc:
     d.g = g
     d.shape_factor = int(static.shape_factor)
     d.relativistic = 1 if static.relativistic else 0
+    if static.particle_pusher not in N.PUSHERS:
+        raise ValueError(f"Unknown particle_pusher: {static.particle_pusher}")
+    d.pusher = N.PUSHERS[static.particle_pusher]
     for a in range(3):
         d.bc[a] = int(static.boundary_conditions[a])
         d.pbc[a] = int(static.particle_boundary_conditions[a])

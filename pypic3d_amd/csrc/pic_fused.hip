@@ -288,7 +288,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_push_esirkepov_v2(
             // ---- push (boris.py:14-124), masked by update_u (particle_push.py:134-142)
             float nv[3] = {v[0], v[1], v[2]};
             float h = (float)(sp.charge[s] * d.dt / (2.0 * sp.mass[s]));
-            boris_push(nv, E, B, h, d.relativistic, (float)d.C);
+            push_velocity(nv, E, B, h, d);
 #pragma unroll
             for (int a = 0; a < 3; ++a)
                 if (sp.update_u[s][a]) v[a] = nv[a];

@@ -125,6 +125,42 @@ __device__ inline void boris_push(float v[3], const float E[3], const float B[3]
     }
 }
 
+// Higuera-Cary (higuera_cary.py:57-113): u = gamma v, u_e = u + hE,
+// gamma_hc = sqrt((sigma + sqrt(sigma^2 + 4 (beta^2 + u*^2))) / 2) with
+// sigma = (1 + u_e^2/C^2) - beta^2, u* = u_e.beta / C, beta = hB, t = beta / gamma_hc,
+// u_m = (u_e + (u_e.t) t + u_e x t) / (1 + t^2), u = u_m + hE + u_m x t.
+__device__ inline void higuera_cary_push(float v[3], const float E[3], const float B[3], float h, float C) {
+    const float invC2 = __builtin_amdgcn_rcpf(C * C);
+    float v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    float gamma0 = __builtin_amdgcn_rsqf(1.0f - v2 * invC2);
+    float ue[3], beta[3], t[3], um[3], c1[3];
+    for (int a = 0; a < 3; ++a) {
+        ue[a] = v[a] * gamma0 + h * E[a];
+        beta[a] = h * B[a];
+    }
+    float b2 = beta[0] * beta[0] + beta[1] * beta[1] + beta[2] * beta[2];
+    float ustar = (ue[0] * beta[0] + ue[1] * beta[1] + ue[2] * beta[2]) / C;
+    float sigma = (1.0f + (ue[0] * ue[0] + ue[1] * ue[1] + ue[2] * ue[2]) * invC2) - b2;
+    float gam = sqrtf(0.5f * (sigma + sqrtf(sigma * sigma + 4.0f * (b2 + ustar * ustar))));
+    float ig = __builtin_amdgcn_rcpf(gam);
+    for (int a = 0; a < 3; ++a) t[a] = beta[a] * ig;
+    float s = __builtin_amdgcn_rcpf(1.0f + t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    float uet = ue[0] * t[0] + ue[1] * t[1] + ue[2] * t[2];
+    cross(ue, t, c1);
+    for (int a = 0; a < 3; ++a) um[a] = s * (ue[a] + uet * t[a] + c1[a]);
+    cross(um, t, c1);
+    float nu[3];
+    for (int a = 0; a < 3; ++a) nu[a] = um[a] + h * E[a] + c1[a];
+    float ing = __builtin_amdgcn_rsqf(1.0f + (nu[0] * nu[0] + nu[1] * nu[1] + nu[2] * nu[2]) * invC2);
+    for (int a = 0; a < 3; ++a) v[a] = nu[a] * ing;
+}
+
+// particle_push.py:109-132 dispatch
+__device__ inline void push_velocity(float v[3], const float E[3], const float B[3], float h, const PicDesc& d) {
+    if (d.pusher == 1) higuera_cary_push(v, E, B, h, (float)d.C);
+    else boris_push(v, E, B, h, d.relativistic, (float)d.C);
+}
+
 // ------------------------------------------------------------------------
 // Esirkepov (Esirkepov.py:105-331, 365-504)
 // ------------------------------------------------------------------------

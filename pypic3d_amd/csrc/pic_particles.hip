@@ -34,7 +34,7 @@ __global__ void push_kernel(PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs
         float E[3], B[3];
         gather_EB(d, fp, pos, E, B);
         float h = (float)(sp.charge[s] * d.dt / (2.0 * sp.mass[s]));
-        boris_push(nv, E, B, h, d.relativistic, (float)d.C);
+        push_velocity(nv, E, B, h, d);
         for (int a = 0; a < 3; ++a)
             if (!sp.update_u[s][a]) nv[a] = v[a];
     }

@@ -3,7 +3,8 @@
 One HIP kernel gathers the six Yee-staggered components with the particle
 shape (CIC or TSC, reduced axes collapsed to index g) and applies the
 non-relativistic (boris.py:14-56) or relativistic (boris.py:60-124) Boris
-update.  Only ``particle_pusher == "boris"`` is implemented.
+update, or the Higuera-Cary update (higuera_cary.py:57-113) when
+``particle_pusher == "higuera_cary"`` (particle_push.py:109-132).
 """
 
 from __future__ import annotations
@@ -14,7 +15,7 @@ from ..particles.particle_class import TiledParticles
 
 
 def check_pusher(static_parameters):
-    if static_parameters.particle_pusher != "boris":
+    if static_parameters.particle_pusher not in ("boris", "higuera_cary"):
         raise ValueError(f"Unknown particle_pusher: {static_parameters.particle_pusher}")
 
 

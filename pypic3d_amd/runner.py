@@ -30,7 +30,7 @@ class ElectrodynamicRunner:
         E, B, J, rho, phi, external_fields, pml_state, overflow = fields
         if pml_state is not None:
             raise NotImplementedError("PML is not implemented on the GPU path")
-        if static_parameters.particle_pusher != "boris":
+        if static_parameters.particle_pusher not in ("boris", "higuera_cary"):
             raise ValueError(f"Unknown particle_pusher: {static_parameters.particle_pusher}")
         self.static, self.dynamic = static_parameters, dynamic_parameters
         self.desc = make_desc(static_parameters, dynamic_parameters)

@@ -34,11 +34,11 @@ DIMS = [(8, 1, 1), (1, 8, 1), (1, 1, 8), (8, 8, 1), (8, 1, 8), (1, 8, 8), (6, 6,
 
 
 def _setup(N, sf=1, rel=True, dep="esirkepov", filt="none", bc=(0, 0, 0), pbc=(0, 0, 0), alpha=1.0, wind=None,
-           electrostatic=False):
+           electrostatic=False, pusher="boris"):
     wind = wind or tuple(0.5 * n for n in N)
     static, dyn = workloads.make_parameters(N, wind=wind, dt=0.05, shape_factor=sf, relativistic=rel,
                                             current_deposition=dep, current_filter=filt, bc=bc, pbc=pbc,
-                                            alpha=alpha, electrostatic=electrostatic)
+                                            alpha=alpha, electrostatic=electrostatic, particle_pusher=pusher)
     return static, dyn, oracle_params(static, dyn)
 
 
@@ -127,6 +127,22 @@ def test_push_gather_boris(N, sf, rel):
     want = O.particle_push(x, u, act, sp, E, B, p)
     assert rel_err(host(out.u[0, 0, 0]), want) <= TOL
     assert out.x is parts.x and out.active is parts.active
+
+
+@pytest.mark.parametrize("N", DIMS)
+@pytest.mark.parametrize("sf", [1, 2])
+def test_push_gather_higuera_cary(N, sf):
+    # pusher_test.py:240-291: particle_push with particle_pusher="higuera_cary"
+    static, dyn, p = _setup(N, sf=sf, pusher="higuera_cary")
+    x, u, act = random_particles(p, S=2, P=300, seed=sum(N) + 3 * sf, vmax=0.3, inactive_frac=0.1)
+    sp = species(2)
+    sp.update_u[0, 2] = False
+    E = [random_field(p, 40 + c, scale=0.5) for c in range(3)]
+    B = [random_field(p, 50 + c, scale=0.5) for c in range(3)]
+    parts = tiled_particles_from_arrays(x, u, act)
+    out = particle_push(parts, _cfg(sp), tuple(map(gpu_field, E)), tuple(map(gpu_field, B)), static, dyn)
+    want = O.particle_push(x, u, act, sp, E, B, p)
+    assert rel_err(host(out.u[0, 0, 0]), want) <= TOL
 
 
 @pytest.mark.parametrize("N", DIMS)

@@ -154,11 +154,12 @@ def test_esirkepov_continuity_and_total_current_at_size(N, ppc):
             assert abs(Jd[a][A].sum().item() - tot) <= 1e-5 * scale
 
 
-@pytest.mark.parametrize("N,ppc", [((12, 10, 8), 24), ((9, 7, 1), 40), ((6, 5, 4), 70)])
-def test_fused_step_on_cell_sorted_dense_plasma_matches_oracle(N, ppc):
+@pytest.mark.parametrize("N,ppc,pusher", [((12, 10, 8), 24, "boris"), ((9, 7, 1), 40, "boris"),
+                                          ((6, 5, 4), 70, "boris"), ((12, 10, 8), 24, "higuera_cary")])
+def test_fused_step_on_cell_sorted_dense_plasma_matches_oracle(N, ppc, pusher):
     """Cell-sorted particles with long equal-cell runs exercise the wave
     segmented reduction of the fused deposit (runs longer than a wave too)."""
-    static, dyn = workloads.make_parameters(N, cfl=0.5)
+    static, dyn = workloads.make_parameters(N, cfl=0.5, particle_pusher=pusher)
     p = oracle_params(static, dyn)
     specs = [workloads.SpeciesSpec(ppc, weight=0.01, vth=(0.08,) * 3),
              workloads.SpeciesSpec(ppc // 2, charge=1.0, mass=50.0, weight=0.01, vth=(0.02,) * 3)]

@@ -186,3 +186,96 @@ def test_particle_push_respects_active_and_update_flags():
     assert np.array_equal(out[0, 3], u[0, 3])
     assert np.array_equal(out[1, :, 1], u[1, :, 1])
     assert not np.allclose(out[1, :, 0], u[1, :, 0])
+
+
+def _hc_step(v, E, B, dt, steps, C=1.0, Efun=None, q=1.0, m=1.0):
+    v = np.asarray(v, dtype=np.float64).reshape(3, 1)
+    for k in range(steps):
+        e = Efun(k) if Efun else E
+        v = O.higuera_cary(v, np.asarray(e, dtype=np.float64).reshape(3, 1),
+                           np.asarray(B, dtype=np.float64).reshape(3, 1), q, m, dt, C)
+    return v[:, 0]
+
+
+def test_higuera_cary_schmitz_constant_B_preserves_energy():
+    # pusher_schmitz_test.py:69-84 (Higuera-Cary half)
+    for gamma0 in (1.001, 10.0):
+        u0 = math.sqrt(gamma0 ** 2 - 1.0)
+        v = _hc_step((u0 / gamma0, 0.0, 0.0), (0, 0, 0), (0, 0, 1.0), 0.2 * math.pi * gamma0, 10)
+        assert abs(_gamma(v) - gamma0) < 2e-12
+
+
+def test_higuera_cary_schmitz_force_free_is_exact():
+    # pusher_schmitz_test.py:85-120: E = -v0 x B keeps the straight orbit to
+    # machine precision (a dropped or mis-signed term in the gamma expression
+    # or the rotation breaks this by many orders of magnitude).
+    gamma0 = 10.0
+    v0 = math.sqrt(gamma0 ** 2 - 1.0) / gamma0
+    E, B, dt = (0.0, v0, 0.0), (0.0, 0.0, 1.0), 0.2 * math.pi * gamma0
+    v = np.array([v0, 0.0, 0.0])
+    ang = en = 0.0
+    for _ in range(20):
+        v = _hc_step(v, E, B, dt, 1)
+        ang = max(ang, abs(math.atan2(v[1], v[0])))
+        en = max(en, abs((_gamma(v) - gamma0) / (gamma0 - 1.0)))
+    assert ang < 1e-13 and en < 1e-12
+
+
+def test_higuera_cary_schmitz_boosted_gyration():
+    # pusher_schmitz_test.py:120-183 (Higuera-Cary bound 1e-6)
+    gp, gf = 10.0, 5.0
+    bf = math.sqrt(1.0 - 1.0 / gf ** 2)
+    vp = math.sqrt(1.0 - 1.0 / gp ** 2)
+    dt = 1.0e-3 * 2.0 * math.pi * gp * gf
+    v = _hc_step((bf, vp / gf, 0.0), (0.0, bf * gf, 0.0), (0.0, 0.0, gf), dt, 20)
+    up = math.sqrt(gp ** 2 - 1.0)
+    omega, radius = 1.0 / gp, (up / gp) / (1.0 / gp)
+    t_lab = 20 * dt
+    tm = t_lab / gf
+    for _ in range(8):
+        res = gf * (tm + bf * radius * (1.0 - math.cos(omega * tm))) - t_lab
+        der = gf * (1.0 + bf * vp * math.sin(omega * tm))
+        tm -= res / der
+    vxm, vym = vp * math.sin(omega * tm), vp * math.cos(omega * tm)
+    den = 1.0 + bf * vxm
+    exact = np.array([(vxm + bf) / den, vym / (gf * den), 0.0])
+    assert np.linalg.norm(v - exact) < 1e-6
+
+
+def test_higuera_cary_schmitz_oscillating_parallel_E():
+    # pusher_schmitz_test.py:185-211 (Higuera-Cary half)
+    gperp = 1.1
+    up = math.sqrt(gperp ** 2 - 1.0)
+    omega0 = 0.5 * (1.0 / gperp)
+    E0 = 10.0 * omega0
+    dt = (2 * math.pi / omega0) / 100
+    v = _hc_step((up / gperp, 0.0, 0.0), None, (0, 0, 1.0), dt, 500,
+                 Efun=lambda k: (0.0, 0.0, E0 * math.cos(omega0 * (k + 0.5) * dt)))
+    assert abs(v[2]) < 1e-10 and abs(_gamma(v) - gperp) < 1e-10
+
+
+def test_higuera_cary_single_particle_known_answers():
+    # boris_test.py:185-247 with C = 10.
+    C = 10.0
+    v = _hc_step((0.2, -0.1, 0.05), (0, 0, 0), (0, 0, 0), 0.1, 1, C=C)
+    np.testing.assert_allclose(v, [0.2, -0.1, 0.05], rtol=1e-14)
+    v = _hc_step((0.2, 0.0, 0.0), (0, 0, 0), (0, 0, 1.0), 0.1, 1, C=C)
+    assert abs(np.linalg.norm(v) - 0.2) < 1e-12 and v[1] < 0
+    v = _hc_step((0.0, 0.0, 0.0), (1.0, 0, 0), (0, 0, 0), 0.1, 1, C=C)
+    assert v[0] > 0 and v[1] == 0 and v[2] == 0
+    # pure E from rest: u = q E dt / m exactly, v = u / sqrt(1 + u^2/C^2)
+    np.testing.assert_allclose(v[0], 0.1 / math.sqrt(1 + 0.01 / C ** 2), rtol=1e-15)
+    q = np.array([1.0, 2.0])
+    out = O.higuera_cary(np.zeros((3, 2)), np.array([[1.0, 1.0], [0, 0], [0, 0]]), np.zeros((3, 2)),
+                         q, np.ones(2), 0.1, C)
+    assert out[0, 1] > out[0, 0] and not out[1:].any()
+
+
+def test_higuera_cary_nonrelativistic_limit_matches_boris():
+    # As C -> inf both relativistic maps reduce to the classical Boris rotation.
+    rng = np.random.default_rng(9)
+    v = rng.standard_normal((3, 16)) * 0.1
+    E = rng.standard_normal((3, 16)); B = rng.standard_normal((3, 16))
+    a = O.higuera_cary(v, E, B, -1.0, 1.0, 0.05, 1e8)
+    b = O.boris(v, E, B, -1.0, 1.0, 0.05)
+    np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12)
