@@ -51,6 +51,61 @@ def cpu_baseline(n=32, ppc=16):
                       f"({P} particles), {dt:.1f} s"}
 
 
+def _timed(fn, reps):
+    import torch
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def extras(n, ppc, steps, seed):
+    """The other deposition paths of the same workload: time_loop_electrodynamic
+    with J_from_rhov (filter "none" and "bilinear"), compute_rho, and the
+    electrostatic CG per iteration (FP32 vectors, FP64 dots)."""
+    import torch
+    from pypic3d_amd import _native as N
+    from pypic3d_amd import workloads
+    from pypic3d_amd.deposition.rho import deposit_rho_into
+    from pypic3d_amd.runner import ElectrodynamicRunner
+    from pypic3d_amd.solvers.electrostatic_yee import last_solve_info, poisson_cg_into
+
+    out = {}
+    n_cells = n ** 3
+    for filt in ("none", "bilinear"):
+        static, dyn, species = workloads.config_warm_3d(n=n, ppc=ppc, vth=0.05, current_deposition="j_from_rhov")
+        static = static._replace(current_filter=filt)
+        parts, cfg = workloads.plasma(static, dyn, species, seed=seed)
+        r = ElectrodynamicRunner(parts, cfg, workloads.empty_fields(static, dyn), static, dyn, sort_every=10)
+        del parts
+        ms = _timed(r.step, steps)
+        npart = r.S * r.P
+        out[f"j_from_rhov_{filt}"] = {"ms_per_step": ms, "particle_steps_per_s": npart / (ms * 1e-3)}
+        if filt == "none":
+            rho = torch.empty_like(r.J[0])
+            ms_rho = _timed(lambda: deposit_rho_into(r.desc, r.sp, r._pp(r.cur), rho, "none", N.stream_handle()), 3)
+            out["rho"] = {"ms": ms_rho, "particles_per_s": npart / (ms_rho * 1e-3),
+                          "algorithmic_bytes": 13 * npart + 4 * n_cells}
+            phi = torch.zeros_like(rho)
+            iters = 50
+
+            def cg():
+                phi.zero_()
+                poisson_cg_into(r.desc, rho, phi, 0.0, iters, N.stream_handle(), rtol=0.0)
+            ms_cg = _timed(cg, 2)
+            k = last_solve_info["iterations"] or iters
+            out["poisson_cg"] = {"iterations": k, "ms_per_iteration": ms_cg / k,
+                                 "hbm_gbs_at_40B_per_cell_iter": 40 * n_cells * k / (ms_cg * 1e-3) / 1e9}
+        del r
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +116,8 @@ def main():
     ap.add_argument("--sort-every", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the J_from_rhov / rho / CG measurements of the same workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="only run a plain device copy of MiB MiB five times (counter calibration)")
     args = ap.parse_args()
@@ -149,6 +206,11 @@ def main():
         "step_hbm_gbs": step_bytes * args.steps / elapsed / 1e9,
         "phase_ms": phases,
     }
+    if not args.no_extras:
+        del runner
+        torch.cuda.empty_cache()
+        print("[bench] J_from_rhov, rho and CG on the same workload", file=sys.stderr, flush=True)
+        out["other_paths"] = extras(args.n, args.ppc, 3, 20261202)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (NumPy oracle sample)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()

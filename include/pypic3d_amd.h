@@ -173,6 +173,37 @@ int pic_energy(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in
                const float* const E[3], const float* const B[3], double* out,
                void* workspace, int64_t workspace_bytes, void* stream);
 
+/* ---- adjoint of the electrodynamic Esirkepov step (periodic fields and
+ * particles; the reference has no gradient code -- these are the reverse-mode
+ * transposes of evolve.py:16-103 as computed by the entry points above).
+ * Adjoint arrays are FP64.  "interior" arrays are (Nx, Ny, Nz), z fastest;
+ * "ghosted" arrays have the field layout. ------------------------------- */
+int pic_fill_f64(double* ptr, int64_t n, double value, void* stream);
+/* ghosted[r] = interior[periodic partner of r]: transpose of fold + refresh
+ * (ghost_cells.py:142-289) for the adjoint of J. */
+int pic_periodic_expand_f64(const PicDesc* desc, const double* interior, double* ghosted, void* stream);
+/* interior[partner(r)] += ghosted[r]: transpose of the ghost refresh read by
+ * the gather, for the adjoint of E and B. */
+int pic_periodic_fold_add_f64(const PicDesc* desc, const double* ghosted, double* interior, void* stream);
+/* Transpose of update_B(E) -> update_E(B_h, J) -> update_B(E', filter)
+ * (first_order_yee.py:12-162; evolve.py:86-97) on interior arrays:
+ * in: lamE, lamB = adjoints of the step's E', B'; out (in place): their
+ * adjoints with respect to the step's E, B through the grid update, and
+ * lamJ = adjoint of the folded J.  work: six interior scratch arrays. */
+int pic_adjoint_yee(const PicDesc* desc, double* const lamE[3], double* const lamB[3], double* const lamJ[3],
+                    double* const work[6], void* stream);
+/* Transpose of the particle phase (particle_push.py gather + pusher,
+ * Esirkepov.py deposit, particle_tile_communication.py:82-99 position update)
+ * at the step's start state `state` and fields E, B (+ external):
+ * lam_x, lam_v in = adjoints of x', v'; out (in place) = adjoints of x, v.
+ * lamJ: ghosted expansion of the adjoint of J; lamE, lamB: ghosted FP64
+ * accumulators (zeroed by the caller) receiving the adjoint of the gathered
+ * fields, to be folded with pic_periodic_fold_add_f64. */
+int pic_adjoint_particles(const PicDesc* desc, const PicSpecies* sp, const PicParticles* state,
+                          const float* const E[3], const float* const B[3], const float* const E_ext[3],
+                          const float* const B_ext[3], const double* const lamJ[3], double* const lam_x[3],
+                          double* const lam_v[3], double* const lamE[3], double* const lamB[3], void* stream);
+
 #ifdef __cplusplus
 }
 #endif

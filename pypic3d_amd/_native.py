@@ -83,6 +83,11 @@ SIGNATURES = {
     "pic_efield_from_phi": (c_int, [DP, c_vp, P3, c_vp]),
     "pic_energy_workspace_bytes": (c_i64, [DP, SP]),
     "pic_energy": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, c_i64, c_vp]),
+    "pic_fill_f64": (c_int, [c_vp, c_i64, c_dbl, c_vp]),
+    "pic_periodic_expand_f64": (c_int, [DP, c_vp, c_vp, c_vp]),
+    "pic_periodic_fold_add_f64": (c_int, [DP, c_vp, c_vp, c_vp]),
+    "pic_adjoint_yee": (c_int, [DP, P3, P3, P3, c_vp * 6, c_vp]),
+    "pic_adjoint_particles": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, P3, P3, P3, P3, P3, c_vp]),
 }
 
 _lib = None

@@ -1,0 +1,718 @@
+// Adjoint (reverse-mode) kernels of the electrodynamic Esirkepov step
+// (evolve.py:16-103) for periodic fields and particles.
+//
+// The forward step, per particle and then on the grid:
+//   E_p, B_p = gather(E + E_ext, B + B_ext at x)            (particle_push.py)
+//   v'       = push(v, E_p, B_p)        on update_u axes    (boris.py / higuera_cary.py)
+//   J       += Esirkepov(x, x + v' dt, v'), fold + refresh  (Esirkepov.py)
+//   x'       = x + v' dt                on update_x axes, periodic wrap
+//   B_h = B - (dt/2) curl_f E;  E' = F(E + dt (C^2 curl_b B_h - J/eps));
+//   B' = F(B_h - (dt/2) curl_f E')                          (first_order_yee.py)
+// with curl_f / curl_b the forward / backward difference curls and F the
+// digital filter (identity at alpha = 1).
+//
+// Reverse mode, adjoints in FP64:
+//  * Yee: on interior arrays with periodic wrap the transpose of curl_f is
+//    curl_b and vice versa, and the symmetric digital filter is self-adjoint.
+//  * fold + refresh of J transposes to "read the adjoint at the raw node's
+//    periodic partner": the adjoint of J is expanded to a ghosted array.
+//  * deposit -> gather: each particle reads the adjoint of J on its Esirkepov
+//    stencil and contracts it with dJ/d(x, v'), computed by forward-mode dual
+//    numbers through the same 5-slot Esirkepov algebra.
+//  * gather -> scatter: the adjoint of E_p, B_p is scattered onto the gather
+//    nodes of a ghosted FP64 array (atomics), then folded onto the interior;
+//    dE_p/dx comes from the derivative of the shape weights.
+//  * push: the 3x9 Jacobian of (v, E_p, B_p) -> v' by dual numbers, applied
+//    transposed.
+#include "pic_particle_math.h"
+
+namespace {
+
+using namespace pic;
+
+constexpr int kThreads = 256;
+
+// ------------------------------------------------------------------ duals
+template <int N>
+struct Dual {
+    double v;
+    double t[N];
+};
+
+template <int N>
+__device__ inline Dual<N> dconst(double x) {
+    Dual<N> r;
+    r.v = x;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.t[i] = 0.0;
+    return r;
+}
+
+template <int N>
+__device__ inline Dual<N> dvar(double x, int k) {
+    Dual<N> r = dconst<N>(x);
+    r.t[k] = 1.0;
+    return r;
+}
+
+template <int N>
+__device__ inline Dual<N> operator+(const Dual<N>& a, const Dual<N>& b) {
+    Dual<N> r;
+    r.v = a.v + b.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.t[i] = a.t[i] + b.t[i];
+    return r;
+}
+
+template <int N>
+__device__ inline Dual<N> operator-(const Dual<N>& a, const Dual<N>& b) {
+    Dual<N> r;
+    r.v = a.v - b.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.t[i] = a.t[i] - b.t[i];
+    return r;
+}
+
+template <int N>
+__device__ inline Dual<N> operator*(const Dual<N>& a, const Dual<N>& b) {
+    Dual<N> r;
+    r.v = a.v * b.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.t[i] = a.t[i] * b.v + a.v * b.t[i];
+    return r;
+}
+
+template <int N>
+__device__ inline Dual<N> operator/(const Dual<N>& a, const Dual<N>& b) {
+    Dual<N> r;
+    r.v = a.v / b.v;
+    const double ib = 1.0 / b.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.t[i] = (a.t[i] - r.v * b.t[i]) * ib;
+    return r;
+}
+
+template <int N>
+__device__ inline Dual<N> operator+(const Dual<N>& a, double b) {
+    Dual<N> r = a;
+    r.v += b;
+    return r;
+}
+
+template <int N>
+__device__ inline Dual<N> operator+(double b, const Dual<N>& a) {
+    return a + b;
+}
+
+template <int N>
+__device__ inline Dual<N> operator-(const Dual<N>& a, double b) {
+    Dual<N> r = a;
+    r.v -= b;
+    return r;
+}
+
+template <int N>
+__device__ inline Dual<N> operator-(double b, const Dual<N>& a) {
+    Dual<N> r;
+    r.v = b - a.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.t[i] = -a.t[i];
+    return r;
+}
+
+template <int N>
+__device__ inline Dual<N> operator*(const Dual<N>& a, double b) {
+    Dual<N> r;
+    r.v = a.v * b;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.t[i] = a.t[i] * b;
+    return r;
+}
+
+template <int N>
+__device__ inline Dual<N> operator*(double b, const Dual<N>& a) {
+    return a * b;
+}
+
+template <int N>
+__device__ inline Dual<N> operator/(const Dual<N>& a, double b) {
+    return a * (1.0 / b);
+}
+
+template <int N>
+__device__ inline Dual<N> operator/(double a, const Dual<N>& b) {
+    return dconst<N>(a) / b;
+}
+
+template <int N>
+__device__ inline Dual<N> dsqrt(const Dual<N>& a) {
+    Dual<N> r;
+    r.v = sqrt(a.v);
+    const double k = 0.5 / r.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.t[i] = a.t[i] * k;
+    return r;
+}
+
+template <class T>
+__device__ inline void tcross(const T a[3], const T b[3], T c[3]) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+template <class T>
+__device__ inline T tdot(const T a[3], const T b[3]) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// ---------------------------------------------------------------- pushers
+// boris_single_particle (boris.py:14-56), relativistic_boris_single_particle
+// (boris.py:60-124) and higuera_cary_single_particle (higuera_cary.py:57-113)
+// in FP64 over a generic scalar (the dual numbers give their Jacobians).
+template <class T>
+__device__ inline void push_t(const T v[3], const T E[3], const T B[3], double h, int pusher, int relativistic,
+                              double C, T out[3]) {
+    const double C2 = C * C;
+    if (pusher == 1) {
+        T u[3], ue[3], beta[3], t[3], um[3], c1[3], nu[3];
+        T g0 = 1.0 / dsqrt(1.0 - tdot(v, v) / C2);
+        for (int a = 0; a < 3; ++a) {
+            u[a] = v[a] * g0;
+            ue[a] = u[a] + E[a] * h;
+            beta[a] = B[a] * h;
+        }
+        T b2 = tdot(beta, beta);
+        T ustar = tdot(ue, beta) / C;
+        T sigma = (1.0 + tdot(ue, ue) / C2) - b2;
+        T gam = dsqrt((sigma + dsqrt(sigma * sigma + 4.0 * (b2 + ustar * ustar))) / 2.0);
+        for (int a = 0; a < 3; ++a) t[a] = beta[a] / gam;
+        T s = 1.0 / (1.0 + tdot(t, t));
+        T uet = tdot(ue, t);
+        tcross(ue, t, c1);
+        for (int a = 0; a < 3; ++a) um[a] = s * (ue[a] + uet * t[a] + c1[a]);
+        tcross(um, t, c1);
+        for (int a = 0; a < 3; ++a) nu[a] = um[a] + E[a] * h + c1[a];
+        T ng = dsqrt(1.0 + tdot(nu, nu) / C2);
+        for (int a = 0; a < 3; ++a) out[a] = nu[a] / ng;
+        return;
+    }
+    T um[3], t[3], up[3], s[3], c1[3];
+    T gm;
+    if (relativistic) {
+        T g = 1.0 / dsqrt(1.0 - tdot(v, v) / C2);
+        for (int a = 0; a < 3; ++a) um[a] = v[a] * g + E[a] * h;
+        gm = dsqrt(1.0 + tdot(um, um) / C2);
+        for (int a = 0; a < 3; ++a) t[a] = B[a] * h / gm;
+    } else {
+        for (int a = 0; a < 3; ++a) {
+            um[a] = v[a] + E[a] * h;
+            t[a] = B[a] * h;
+        }
+    }
+    tcross(um, t, c1);
+    for (int a = 0; a < 3; ++a) up[a] = um[a] + c1[a];
+    T tt = 1.0 + tdot(t, t);
+    for (int a = 0; a < 3; ++a) s[a] = 2.0 * t[a] / tt;
+    tcross(up, s, c1);
+    T nu[3];
+    for (int a = 0; a < 3; ++a) nu[a] = um[a] + c1[a] + E[a] * h;
+    if (relativistic) {
+        T ng = dsqrt(1.0 + tdot(nu, nu) / C2);
+        for (int a = 0; a < 3; ++a) out[a] = nu[a] / ng;
+    } else {
+        for (int a = 0; a < 3; ++a) out[a] = nu[a];
+    }
+}
+
+// ----------------------------------------------------------------- gather
+// One axis of interpolate_field_to_particles (boris.py:127-258): stencil
+// points, weights and their x-derivatives (weights divide by the grid spacing).
+struct AdjAxis {
+    int p[3];
+    double w[3], dw[3];
+    int n;
+};
+
+__device__ inline void adj_gather_axis(double x, double org, double sp, int sf, int L, bool active, int g,
+                                       AdjAxis& s) {
+    int a = anchor_of(x, org, sp, sf);
+    double r = (x - node_coord(a, sp, org)) / sp;
+    double w[3], dw[3];
+    if (sf == 1) {
+        w[0] = 0.0; w[1] = 1.0 - r; w[2] = r;
+        dw[0] = 0.0; dw[1] = -1.0 / sp; dw[2] = 1.0 / sp;
+    } else {
+        w[0] = 0.5 * (0.5 - r) * (0.5 - r);
+        w[1] = 0.75 - r * r;
+        w[2] = 0.5 * (0.5 + r) * (0.5 + r);
+        dw[0] = -(0.5 - r) / sp;
+        dw[1] = -2.0 * r / sp;
+        dw[2] = (0.5 + r) / sp;
+    }
+    if (active) {
+        for (int q = 0; q < 3; ++q) {
+            s.p[q] = pmod(a - 1 + q, L);
+            s.w[q] = w[q];
+            s.dw[q] = dw[q];
+        }
+        s.n = 3;
+    } else {
+        s.p[0] = g;
+        s.w[0] = w[0] + w[1] + w[2];
+        s.dw[0] = 0.0;      // the collapsed weight is the partition of unity
+        s.n = 1;
+    }
+}
+
+struct AdjGrid {
+    AdjAxis c[3], v[3];
+};
+
+// value and d/dx of one staggered component; `stag` bit a set = vertex line on axis a
+__device__ inline void gather_grad(const PicDesc& d, const AdjGrid& gr, int stag, const float* F, const float* Fe,
+                                   double& val, double grad[3]) {
+    const AdjAxis& ax = (stag & 1) ? gr.v[0] : gr.c[0];
+    const AdjAxis& ay = (stag & 2) ? gr.v[1] : gr.c[1];
+    const AdjAxis& az = (stag & 4) ? gr.v[2] : gr.c[2];
+    val = 0.0;
+    grad[0] = grad[1] = grad[2] = 0.0;
+    for (int i = 0; i < ax.n; ++i)
+        for (int j = 0; j < ay.n; ++j)
+            for (int k = 0; k < az.n; ++k) {
+                int64_t id = fidx(d, ax.p[i], ay.p[j], az.p[k]);
+                double f = (double)F[id];
+                if (Fe) f += (double)Fe[id];
+                val += f * ax.w[i] * ay.w[j] * az.w[k];
+                grad[0] += f * ax.dw[i] * ay.w[j] * az.w[k];
+                grad[1] += f * ax.w[i] * ay.dw[j] * az.w[k];
+                grad[2] += f * ax.w[i] * ay.w[j] * az.dw[k];
+            }
+}
+
+__device__ inline void scatter_adj(const PicDesc& d, const AdjGrid& gr, int stag, double lam, double* G) {
+    if (lam == 0.0) return;
+    const AdjAxis& ax = (stag & 1) ? gr.v[0] : gr.c[0];
+    const AdjAxis& ay = (stag & 2) ? gr.v[1] : gr.c[1];
+    const AdjAxis& az = (stag & 4) ? gr.v[2] : gr.c[2];
+    for (int i = 0; i < ax.n; ++i)
+        for (int j = 0; j < ay.n; ++j)
+            for (int k = 0; k < az.n; ++k) {
+                double w = ax.w[i] * ay.w[j] * az.w[k];
+                if (w != 0.0) unsafeAtomicAdd(&G[fidx(d, ax.p[i], ay.p[j], az.p[k])], lam * w);
+            }
+}
+
+// Staggering (particle_push.py:63-68): Ex (v,c,c) Ey (c,v,c) Ez (c,c,v)
+// Bx (c,v,v) By (v,c,v) Bz (v,v,c); bit a = vertex on axis a.
+__device__ constexpr int kStagE[3] = {1, 2, 4};
+__device__ constexpr int kStagB[3] = {6, 5, 3};
+
+// -------------------------------------------------------------- Esirkepov
+// Esirkepov.py:105-331 over dual numbers with tangents (x_old[3], v'[3]).
+using D6 = Dual<6>;
+
+struct DualAxis {
+    int base;
+    D6 S0[5], D[5], C[5];
+};
+
+__device__ inline void shape3_dual(const D6& r, int sf, D6 w[3]) {
+    if (sf == 1) {
+        w[0] = dconst<6>(0.0);
+        w[1] = 1.0 - r;
+        w[2] = r;
+    } else {
+        D6 a = 0.5 - r, b = 0.5 + r;
+        w[0] = 0.5 * (a * a);
+        w[1] = 0.75 - r * r;
+        w[2] = 0.5 * (b * b);
+    }
+}
+
+__device__ inline void dual_axis(const D6& xo, const D6& xn, double org, double sp, double dd, int sf, bool active,
+                                 int g, DualAxis& e) {
+    int ao = anchor_of(xo.v, org, sp, sf);
+    int an = anchor_of(xn.v, org, sp, sf);
+    D6 wo[3], wn[3];
+    shape3_dual((xo - node_coord(ao, sp, org)) / dd, sf, wo);
+    shape3_dual((xn - node_coord(an, sp, org)) / dd, sf, wn);
+    D6 zero = dconst<6>(0.0);
+    D6 S1[5] = {zero, wn[0], wn[1], wn[2], zero};
+    D6 Sp[5] = {zero, wo[0], wo[1], wo[2], zero};
+    int shift = an - ao;
+    D6 run = zero;
+    for (int i = 0; i < 5; ++i) {
+        e.S0[i] = Sp[pmod(i + shift, 5)];          // jnp.roll(w5, -shift)
+        e.D[i] = S1[i] - e.S0[i];
+    }
+    if (!active) {
+        D6 tn = S1[0] + S1[1] + S1[2] + S1[3] + S1[4];
+        D6 to = e.S0[0] + e.S0[1] + e.S0[2] + e.S0[3] + e.S0[4];
+        e.base = g;
+        e.S0[0] = to;
+        e.D[0] = tn - to;
+        e.C[0] = zero;
+        return;
+    }
+    e.base = an - 2;
+    for (int i = 0; i < 5; ++i) {
+        run = run + e.D[i];
+        e.C[i] = run;
+    }
+}
+
+// Contracts lamJ (ghosted, the adjoint of the raw deposit) with dJ/d(x_old, v').
+__device__ inline void esirkepov_adjoint(const PicDesc& d, double qw, const D6 xo[3], const D6 xn[3], const D6 v[3],
+                                         const double* const lamJ[3], double g6[6]) {
+    DualAxis ax[3];
+    bool act[3];
+    for (int a = 0; a < 3; ++a) {
+        act[a] = d.n[a] > 1;
+        dual_axis(xo[a], xn[a], d.org_c[a], d.sp_c[a], d.d[a], d.shape_factor, act[a], d.g, ax[a]);
+    }
+    const double dV = d.d[0] * d.d[1] * d.d[2];
+    for (int c = 0; c < 3; ++c) {
+        int o1 = c == 0 ? 1 : 0, o2 = c == 2 ? 1 : 2;
+        int fa[2], nf = 0;
+        for (int a = 0; a < 3; ++a)
+            if (a != c && act[a]) fa[nf++] = a;
+        D6 pre;
+        int ni;
+        if (act[c]) {
+            pre = dconst<6>(-(qw / (d.d[o1] * d.d[o2])) / d.dt);
+            ni = 5;
+        } else {
+            pre = v[c] * (qw / dV);
+            ni = 1;
+        }
+        const int nj = nf >= 1 ? 5 : 1, nk = nf == 2 ? 5 : 1;
+        const double* L = lamJ[c];
+        for (int i = 0; i < ni; ++i) {
+            D6 ci = act[c] ? ax[c].C[i] : dconst<6>(1.0);
+            for (int j = 0; j < nj; ++j)
+                for (int k = 0; k < nk; ++k) {
+                    int idx[3] = {ax[0].base, ax[1].base, ax[2].base};
+                    if (act[c]) idx[c] += i;
+                    if (nf >= 1) idx[fa[0]] += j;
+                    if (nf == 2) idx[fa[1]] += k;
+                    if (!drop_index(idx[0], d.dims[0]) || !drop_index(idx[1], d.dims[1]) ||
+                        !drop_index(idx[2], d.dims[2]))
+                        continue;
+                    double lam = L[fidx(d, idx[0], idx[1], idx[2])];
+                    if (lam == 0.0) continue;
+                    D6 y;
+                    if (nf == 0) {
+                        y = dconst<6>(1.0);
+                    } else if (nf == 1) {
+                        const DualAxis& A = ax[fa[0]];
+                        y = A.S0[j] + 0.5 * A.D[j];
+                    } else {
+                        const DualAxis& A = ax[fa[0]];
+                        const DualAxis& B = ax[fa[1]];
+                        y = A.S0[j] * B.S0[k] + 0.5 * (A.S0[j] * B.D[k] + A.D[j] * B.S0[k]) +
+                            (1.0 / 3.0) * (A.D[j] * B.D[k]);
+                    }
+                    D6 val = pre * ci * y;
+                    for (int m = 0; m < 6; ++m) g6[m] += lam * val.t[m];
+                }
+        }
+    }
+}
+
+// ----------------------------------------------------------- particle kernel
+struct LamPtrs {
+    const double* J[3];
+    double* x[3];
+    double* v[3];
+    double* E[3];
+    double* B[3];
+};
+
+__global__ void adjoint_particles_kernel(PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, LamPtrs lp,
+                                         int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!in.active[i]) {
+        for (int a = 0; a < 3; ++a) {
+            lp.x[a][i] = 0.0;
+            lp.v[a][i] = 0.0;
+        }
+        return;
+    }
+    const int s = species_of(i, sp.slots, sp.count);
+    const SpeciesK k = species_constants(d, sp, s);
+    double x[3], v[3], lx[3], lv[3];
+    for (int a = 0; a < 3; ++a) {
+        x[a] = (double)in.x[a][i];
+        v[a] = (double)in.u[a][i];
+        lx[a] = lp.x[a][i];          // adjoint of x' (the wrap is the identity map's derivative)
+        lv[a] = lp.v[a][i];          // adjoint of v'
+    }
+    // gather with gradients
+    AdjGrid gr;
+    for (int a = 0; a < 3; ++a) {
+        bool act = d.n[a] > 1;
+        adj_gather_axis(x[a], d.org_c[a], d.sp_c[a], d.shape_factor, d.dims[a], act, d.g, gr.c[a]);
+        adj_gather_axis(x[a], d.org_v[a], d.sp_v[a], d.shape_factor, d.dims[a], act, d.g, gr.v[a]);
+    }
+    double Ep[3], Bp[3], dE[3][3], dB[3][3];
+    for (int c = 0; c < 3; ++c) {
+        gather_grad(d, gr, kStagE[c], fp.E[c], fp.Ee[c], Ep[c], dE[c]);
+        gather_grad(d, gr, kStagB[c], fp.B[c], fp.Be[c], Bp[c], dB[c]);
+    }
+    // push and its Jacobian over (v, E_p, B_p)
+    Dual<9> vv[3], EE[3], BB[3], vn[3];
+    for (int a = 0; a < 3; ++a) {
+        vv[a] = dvar<9>(v[a], a);
+        EE[a] = dvar<9>(Ep[a], 3 + a);
+        BB[a] = dvar<9>(Bp[a], 6 + a);
+    }
+    const double h = sp.charge[s] * d.dt / (2.0 * sp.mass[s]);
+    push_t(vv, EE, BB, h, d.pusher, d.relativistic, d.C, vn);
+    double vp[3];
+    for (int a = 0; a < 3; ++a) vp[a] = ((k.uu >> a) & 1u) ? vn[a].v : v[a];
+    // position update x' = x + v' dt on update_x axes
+    for (int a = 0; a < 3; ++a)
+        if ((k.ux >> a) & 1u) lv[a] += d.dt * lx[a];
+    // deposit -> gather of the adjoint of J
+    D6 xo[3], xnew[3], vd[3];
+    for (int a = 0; a < 3; ++a) {
+        xo[a] = dvar<6>(x[a], a);
+        vd[a] = dvar<6>(vp[a], 3 + a);
+        if ((k.ux >> a) & 1u) {
+            xnew[a] = xo[a] + vd[a] * d.dt;
+            xnew[a].v = step_pos(x[a], vp[a], d.dt);    // same rounding as the forward anchor
+        } else {
+            xnew[a] = xo[a];
+        }
+    }
+    double g6[6] = {0, 0, 0, 0, 0, 0};
+    esirkepov_adjoint(d, k.qw, xo, xnew, vd, lp.J, g6);
+    for (int a = 0; a < 3; ++a) {
+        lx[a] += g6[a];
+        lv[a] += g6[3 + a];
+    }
+    // push transposed: lambda(v, E_p, B_p) = J^T lambda(v') on update_u axes
+    double lin[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < 3; ++a) {
+        if ((k.uu >> a) & 1u) {
+            for (int m = 0; m < 9; ++m) lin[m] += lv[a] * vn[a].t[m];
+        } else {
+            lin[a] += lv[a];
+        }
+    }
+    // gather -> scatter, and the weight derivatives
+    for (int c = 0; c < 3; ++c) {
+        scatter_adj(d, gr, kStagE[c], lin[3 + c], lp.E[c]);
+        scatter_adj(d, gr, kStagB[c], lin[6 + c], lp.B[c]);
+        for (int a = 0; a < 3; ++a) lx[a] += lin[3 + c] * dE[c][a] + lin[6 + c] * dB[c][a];
+    }
+    for (int a = 0; a < 3; ++a) {
+        lp.x[a][i] = lx[a];
+        lp.v[a][i] = lin[a];
+    }
+}
+
+// ------------------------------------------------------------ field kernels
+// Interior arrays (Nx, Ny, Nz), z fastest, periodic wrap.
+__device__ inline int64_t iidx(const PicDesc& d, int i, int j, int k) {
+    return ((int64_t)i * d.n[1] + j) * d.n[2] + k;
+}
+
+__device__ inline int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+struct D3 {
+    const double* p[3];
+};
+struct W3 {
+    double* p[3];
+};
+
+// out = a + coef * curl(b); backward = 0 forward differences (update_B's curl E),
+// 1 backward differences (update_E's curl B).
+__global__ void curl_axpy_kernel(PicDesc d, W3 out, D3 a, D3 b, double coef, int backward) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t tot = (int64_t)d.n[0] * d.n[1] * d.n[2];
+    if (t >= tot) return;
+    int k = (int)(t % d.n[2]);
+    int j = (int)((t / d.n[2]) % d.n[1]);
+    int i = (int)(t / ((int64_t)d.n[2] * d.n[1]));
+    int s = backward ? -1 : 1;
+    // difference along axis ax of component comp: forward (f[+1]-f)/d, backward (f - f[-1])/d
+    auto diff = [&](int comp, int ax) -> double {
+        int ii = i, jj = j, kk = k;
+        if (ax == 0) ii = wrap(i + s, d.n[0]);
+        if (ax == 1) jj = wrap(j + s, d.n[1]);
+        if (ax == 2) kk = wrap(k + s, d.n[2]);
+        double here = b.p[comp][t], there = b.p[comp][iidx(d, ii, jj, kk)];
+        return (backward ? here - there : there - here) / d.d[ax];
+    };
+    double cx = diff(2, 1) - diff(1, 2);
+    double cy = diff(0, 2) - diff(2, 0);
+    double cz = diff(1, 0) - diff(0, 1);
+    double ax0 = a.p[0] ? a.p[0][t] : 0.0, ay0 = a.p[1] ? a.p[1][t] : 0.0, az0 = a.p[2] ? a.p[2][t] : 0.0;
+    out.p[0][t] = ax0 + coef * cx;
+    out.p[1][t] = ay0 + coef * cy;
+    out.p[2][t] = az0 + coef * cz;
+}
+
+// digital_filter (filters.py:234-291) on periodic interior data: centre alpha,
+// six faces (1 - alpha)/6.  Symmetric, hence its own transpose.
+__global__ void filter_f64_kernel(PicDesc d, W3 out, D3 in, double alpha) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t tot = (int64_t)d.n[0] * d.n[1] * d.n[2];
+    if (t >= tot) return;
+    int k = (int)(t % d.n[2]);
+    int j = (int)((t / d.n[2]) % d.n[1]);
+    int i = (int)(t / ((int64_t)d.n[2] * d.n[1]));
+    double f = (1.0 - alpha) / 6.0;
+    for (int c = 0; c < 3; ++c) {
+        const double* p = in.p[c];
+        double s = p[iidx(d, wrap(i - 1, d.n[0]), j, k)] + p[iidx(d, wrap(i + 1, d.n[0]), j, k)] +
+                   p[iidx(d, i, wrap(j - 1, d.n[1]), k)] + p[iidx(d, i, wrap(j + 1, d.n[1]), k)] +
+                   p[iidx(d, i, j, wrap(k - 1, d.n[2]))] + p[iidx(d, i, j, wrap(k + 1, d.n[2]))];
+        out.p[c][t] = alpha * p[t] + f * s;
+    }
+}
+
+__global__ void scale_kernel(W3 out, D3 in, double coef, int64_t n) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    for (int c = 0; c < 3; ++c) out.p[c][t] = coef * in.p[c][t];
+}
+
+// periodic partner (interior index) of a ghosted index along one axis
+__device__ inline int partner(int r, int g, int n) {
+    int i = (r - g) % n;
+    return i < 0 ? i + n : i;
+}
+
+__global__ void expand_kernel(PicDesc d, const double* in, double* out, int64_t tot) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tot) return;
+    int k = (int)(t % d.dims[2]);
+    int j = (int)((t / d.dims[2]) % d.dims[1]);
+    int i = (int)(t / ((int64_t)d.dims[2] * d.dims[1]));
+    out[t] = in[iidx(d, partner(i, d.g, d.n[0]), partner(j, d.g, d.n[1]), partner(k, d.g, d.n[2]))];
+}
+
+__global__ void fold_add_kernel(PicDesc d, const double* in, double* out, int64_t tot) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tot) return;
+    double v = in[t];
+    if (v == 0.0) return;
+    int k = (int)(t % d.dims[2]);
+    int j = (int)((t / d.dims[2]) % d.dims[1]);
+    int i = (int)(t / ((int64_t)d.dims[2] * d.dims[1]));
+    unsafeAtomicAdd(&out[iidx(d, partner(i, d.g, d.n[0]), partner(j, d.g, d.n[1]), partner(k, d.g, d.n[2]))], v);
+}
+
+__global__ void fill_f64_kernel(double* p, int64_t n, double v) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+inline bool periodic_all(const PicDesc* d) {
+    for (int a = 0; a < 3; ++a)
+        if (d->bc[a] != 0 || d->pbc[a] != 0) return false;
+    return true;
+}
+
+inline int64_t interior_size(const PicDesc& d) { return (int64_t)d.n[0] * d.n[1] * d.n[2]; }
+
+}  // namespace
+
+extern "C" {
+
+int pic_fill_f64(double* ptr, int64_t n, double value, void* stream) {
+    PIC_REQUIRE(ptr || n == 0);
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(fill_f64_kernel, dim3(grid_1d(n, kThreads)), dim3(kThreads), 0, as_stream(stream), ptr, n,
+                       value);
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+int pic_periodic_expand_f64(const PicDesc* desc, const double* interior, double* ghosted, void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && periodic_all(desc) && interior && ghosted && (const double*)ghosted != interior);
+    int64_t tot = field_size(*desc);
+    hipLaunchKernelGGL(expand_kernel, dim3(grid_1d(tot, kThreads)), dim3(kThreads), 0, as_stream(stream), *desc,
+                       interior, ghosted, tot);
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+int pic_periodic_fold_add_f64(const PicDesc* desc, const double* ghosted, double* interior, void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && periodic_all(desc) && interior && ghosted && ghosted != (const double*)interior);
+    int64_t tot = field_size(*desc);
+    hipLaunchKernelGGL(fold_add_kernel, dim3(grid_1d(tot, kThreads)), dim3(kThreads), 0, as_stream(stream), *desc,
+                       ghosted, interior, tot);
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+int pic_adjoint_yee(const PicDesc* desc, double* const lamE[3], double* const lamB[3], double* const lamJ[3],
+                    double* const work[6], void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && periodic_all(desc) && lamE && lamB && lamJ && work);
+    for (int a = 0; a < 3; ++a) PIC_REQUIRE(lamE[a] && lamB[a] && lamJ[a]);
+    for (int a = 0; a < 6; ++a) PIC_REQUIRE(work[a]);
+    const PicDesc& d = *desc;
+    hipStream_t st = as_stream(stream);
+    const int64_t tot = interior_size(d);
+    const dim3 grid(grid_1d(tot, kThreads)), blk(kThreads);
+    const double half = 0.5 * d.dt;
+    W3 LE{{lamE[0], lamE[1], lamE[2]}}, LB{{lamB[0], lamB[1], lamB[2]}}, LJ{{lamJ[0], lamJ[1], lamJ[2]}};
+    W3 MU{{work[0], work[1], work[2]}}, NU{{work[3], work[4], work[5]}};
+    auto cd = [](const W3& w) { return D3{{w.p[0], w.p[1], w.p[2]}}; };
+    const bool filt = d.alpha != 1.0;
+    // mu = F^T lamB'  (last half step's filter)
+    if (filt) {
+        hipLaunchKernelGGL(filter_f64_kernel, grid, blk, 0, st, d, MU, cd(LB), d.alpha);
+    } else {
+        hipLaunchKernelGGL(scale_kernel, grid, blk, 0, st, MU, cd(LB), 1.0, tot);
+    }
+    // lamE'' = lamE' - (dt/2) curl_f^T mu = lamE' - (dt/2) curl_b mu
+    hipLaunchKernelGGL(curl_axpy_kernel, grid, blk, 0, st, d, LE, cd(LE), cd(MU), -half, 1);
+    // nu = F^T lamE''
+    if (filt) {
+        hipLaunchKernelGGL(filter_f64_kernel, grid, blk, 0, st, d, NU, cd(LE), d.alpha);
+    } else {
+        hipLaunchKernelGGL(scale_kernel, grid, blk, 0, st, NU, cd(LE), 1.0, tot);
+    }
+    // lamJ = -(dt/eps) nu
+    hipLaunchKernelGGL(scale_kernel, grid, blk, 0, st, LJ, cd(NU), -d.dt / d.eps, tot);
+    // lamB_h = mu + dt C^2 curl_b^T nu = mu + dt C^2 curl_f nu
+    hipLaunchKernelGGL(curl_axpy_kernel, grid, blk, 0, st, d, LB, cd(MU), cd(NU), d.dt * d.C * d.C, 0);
+    // lamE = nu - (dt/2) curl_b lamB_h;  lamB = lamB_h
+    hipLaunchKernelGGL(curl_axpy_kernel, grid, blk, 0, st, d, LE, cd(NU), cd(LB), -half, 1);
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+int pic_adjoint_particles(const PicDesc* desc, const PicSpecies* sp, const PicParticles* state,
+                          const float* const E[3], const float* const B[3], const float* const E_ext[3],
+                          const float* const B_ext[3], const double* const lamJ[3], double* const lam_x[3],
+                          double* const lam_v[3], double* const lamE[3], double* const lamB[3], void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && periodic_all(desc) && particles_ok(sp, state) && E && B && lamJ && lam_x &&
+                lam_v && lamE && lamB);
+    for (int a = 0; a < 3; ++a)
+        PIC_REQUIRE(E[a] && B[a] && lamJ[a] && lam_x[a] && lam_v[a] && lamE[a] && lamB[a]);
+    int64_t n = (int64_t)sp->count * sp->slots;
+    if (n == 0) return 0;
+    FieldPtrs fp = make_fields(E, B, E_ext, B_ext);
+    LamPtrs lp;
+    for (int a = 0; a < 3; ++a) {
+        lp.J[a] = lamJ[a];
+        lp.x[a] = lam_x[a];
+        lp.v[a] = lam_v[a];
+        lp.E[a] = lamE[a];
+        lp.B[a] = lamB[a];
+    }
+    hipLaunchKernelGGL(adjoint_particles_kernel, dim3(grid_1d(n, 128)), dim3(128), 0, as_stream(stream), *desc,
+                       *sp, *state, fp, lp, n);
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+}  // extern "C"

@@ -216,7 +216,9 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_push_esirkepov_v2(
     __shared__ int q_idx[kPerBlock];
     __shared__ float q_v[3][kPerBlock];
     __shared__ int q_n;
+    __shared__ SpeciesK spk[PIC_MAX_SPECIES];
     if (threadIdx.x == 0) q_n = 0;
+    if (threadIdx.x < sp.count) spk[threadIdx.x] = species_constants(d, sp, threadIdx.x);
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
@@ -287,18 +289,18 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_push_esirkepov_v2(
             }
             // ---- push (boris.py:14-124), masked by update_u (particle_push.py:134-142)
             float nv[3] = {v[0], v[1], v[2]};
-            float h = (float)(sp.charge[s] * d.dt / (2.0 * sp.mass[s]));
-            push_velocity(nv, E, B, h, d);
+            const SpeciesK k = spk[s];
+            push_velocity(nv, E, B, k.h, d);
 #pragma unroll
             for (int a = 0; a < 3; ++a)
-                if (sp.update_u[s][a]) v[a] = nv[a];
+                if ((k.uu >> a) & 1u) v[a] = nv[a];
             // ---- new position (Esirkepov.py:125-127) and regular-case test
             double xn[3];
             bool regular = fast3d;
             float S0[3][2], D[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                bool ux = sp.update_x[s][a] != 0;
+                bool ux = (k.ux >> a) & 1u;
                 xn[a] = ux ? step_pos((double)xf[a], (double)v[a], d.dt) : (double)xf[a];
                 if (!regular) continue;
                 float dvt = ux ? v[a] * fdt : 0.0f;
@@ -315,11 +317,10 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_push_esirkepov_v2(
                 // Esirkepov weights in difference form with D = (-D, +D) at (a, a+1):
                 // W = D_c * (S0 S0 + (S0 D + D S0)/2 + D D/3) over the other two axes,
                 // J_c = -(q/(d_o1 d_o2 dt)) * cumsum_c(W) -> value at slot a only.
-                double qw = sp.charge[s] * sp.weight[s];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     int o1 = c == 0 ? 1 : 0, o2 = c == 2 ? 1 : 2;
-                    float pre = (float)(-(qw / (d.d[o1] * d.d[o2])) / d.dt) * (-D[c]);
+                    float pre = k.pre[c] * (-D[c]);
 #pragma unroll
                     for (int j = 0; j < 2; ++j) {
                         float dj = j ? D[o1] : -D[o1];
@@ -401,30 +402,29 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_push_esirkepov_v2(
         // one lane per queued particle, all terms unrolled (no scratch arrays)
         for (int q = threadIdx.x; q < nq; q += kThreads) {
             uint32_t i = base + (uint32_t)q_idx[q];
-            int s = species_of(i, sp.slots, sp.count);
+            const SpeciesK& k = spk[species_of(i, sp.slots, sp.count)];
             float v[3] = {q_v[0][q], q_v[1][q], q_v[2][q]};
             double xo[3], xn[3];
             load_pos(in, i, xo);
 #pragma unroll
-            for (int a = 0; a < 3; ++a) xn[a] = sp.update_x[s][a] ? step_pos(xo[a], (double)v[a], d.dt) : xo[a];
+            for (int a = 0; a < 3; ++a) xn[a] = ((k.ux >> a) & 1u) ? step_pos(xo[a], (double)v[a], d.dt) : xo[a];
             CrossAxis cx, cy, cz;
             bool ok = cross_axis(xo[0], xn[0], d.org_c[0], d.sp_c[0], 1.0 / d.d[0], ac[0].org_c, ac[0].isp_c, cx);
             ok = cross_axis(xo[1], xn[1], d.org_c[1], d.sp_c[1], 1.0 / d.d[1], ac[1].org_c, ac[1].isp_c, cy) && ok;
             ok = cross_axis(xo[2], xn[2], d.org_c[2], d.sp_c[2], 1.0 / d.d[2], ac[2].org_c, ac[2].isp_c, cz) && ok;
-            double qw = sp.charge[s] * sp.weight[s];
-            if (ok) crossing_deposit(d, qw, cx, cy, cz, Jx, Jy, Jz);
-            else esirkepov_particle(d, qw, xo, xn, v, J);   // moved two or more cells
+            if (ok) crossing_deposit(d, k.qw, cx, cy, cz, Jx, Jy, Jz);
+            else esirkepov_particle(d, k.qw, xo, xn, v, J);   // moved two or more cells
         }
     } else {
         for (int q = threadIdx.x; q < nq; q += kThreads) {
             uint32_t i = base + (uint32_t)q_idx[q];
-            int s = species_of(i, sp.slots, sp.count);
+            const SpeciesK& k = spk[species_of(i, sp.slots, sp.count)];
             float v[3] = {q_v[0][q], q_v[1][q], q_v[2][q]};
             double xo[3], xn[3];
             load_pos(in, i, xo);
 #pragma unroll
-            for (int a = 0; a < 3; ++a) xn[a] = sp.update_x[s][a] ? step_pos(xo[a], (double)v[a], d.dt) : xo[a];
-            esirkepov_particle(d, sp.charge[s] * sp.weight[s], xo, xn, v, J);
+            for (int a = 0; a < 3; ++a) xn[a] = ((k.ux >> a) & 1u) ? step_pos(xo[a], (double)v[a], d.dt) : xo[a];
+            esirkepov_particle(d, k.qw, xo, xn, v, J);
         }
     }
 }

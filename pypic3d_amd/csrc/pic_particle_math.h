@@ -62,6 +62,35 @@ struct FieldPtrs {
     const float* Be[3];
 };
 
+// Per-species constants of the particle kernels, evaluated once per block in
+// FP64 with the same expressions the per-particle code used, so the FP32
+// values handed to the particle math are unchanged:
+//   qw  = charge * weight                              (Esirkepov.py:102)
+//   h   = charge * dt / (2 mass)                       (boris.py:14-124)
+//   pre = -(qw / (d_o1 d_o2)) / dt per current axis    (Esirkepov.py:197-214)
+//   ux, uu = update_x / update_u as bit masks          (particle_class.py:5-14)
+struct SpeciesK {
+    double qw;
+    float h;
+    float pre[3];
+    uint32_t ux, uu;
+};
+
+__device__ inline SpeciesK species_constants(const PicDesc& d, const PicSpecies& sp, int s) {
+    SpeciesK k;
+    k.qw = sp.charge[s] * sp.weight[s];
+    k.h = (float)(sp.charge[s] * d.dt / (2.0 * sp.mass[s]));
+    k.pre[0] = (float)(-(k.qw / (d.d[1] * d.d[2])) / d.dt);
+    k.pre[1] = (float)(-(k.qw / (d.d[0] * d.d[2])) / d.dt);
+    k.pre[2] = (float)(-(k.qw / (d.d[0] * d.d[1])) / d.dt);
+    k.ux = k.uu = 0u;
+    for (int a = 0; a < 3; ++a) {
+        k.ux |= (sp.update_x[s][a] ? 1u : 0u) << a;
+        k.uu |= (sp.update_u[s][a] ? 1u : 0u) << a;
+    }
+    return k;
+}
+
 // Six staggered components (particle_push.py:63-68): Ex (v,c,c), Ey (c,v,c),
 // Ez (c,c,v), Bx (c,v,v), By (v,c,v), Bz (v,v,c).
 __device__ inline void gather_EB(const PicDesc& d, const FieldPtrs& fp, const double pos[3], float E[3],

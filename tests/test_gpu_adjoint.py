@@ -1,0 +1,161 @@
+"""Adjoint kernels: exact transposes (dot-product tests against the oracle's
+forward maps) and gradients of the final field energy against central finite
+differences of the FP64 oracle (h = 1e-4)."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import pic_oracle as O
+from pypic3d_amd import _native as N
+from pypic3d_amd import workloads
+from pypic3d_amd._tensors import make_desc
+from pypic3d_amd.adjoint import AdjointRun, field_energy_loss, two_stream_state
+from tests.gpu_helpers import oracle_params, oracle_species
+
+pytestmark = pytest.mark.gpu
+
+
+def _interior(f, p):
+    g = p.g
+    return f[g:g + p.N[0], g:g + p.N[1], g:g + p.N[2]]
+
+
+def _ghosted(x, p):
+    """Interior values -> ghosted array with periodic ghosts (oracle refresh)."""
+    f = np.zeros(p.shape)
+    g = p.g
+    f[g:g + p.N[0], g:g + p.N[1], g:g + p.N[2]] = x
+    return O.refresh(f, p)
+
+
+def _p3(t):
+    return N.P3(*[t[c].data_ptr() for c in range(3)])
+
+
+@pytest.mark.parametrize("N_,alpha", [((8, 6, 5), 1.0), ((9, 7, 1), 1.0), ((10, 1, 1), 1.0), ((6, 5, 4), 0.6),
+                                      ((8, 1, 6), 0.7)])
+def test_adjoint_yee_is_the_transpose_of_the_oracle_step(N_, alpha):
+    static, dyn = workloads.make_parameters(N_, wind=tuple(0.7 * n for n in N_), dt=0.05, alpha=alpha)
+    p = oracle_params(static, dyn)
+    desc = make_desc(static, dyn)
+    rng = np.random.default_rng(sum(N_))
+    z = {k: [rng.standard_normal(N_) for _ in range(3)] for k in "EBJ"}
+    lam = {k: [rng.standard_normal(N_) for _ in range(3)] for k in "EB"}
+    # forward (linear in E, B, J): B_h = update_B(E); E' = update_E(B_h, J); B' = update_B(E', filter)
+    E = [_ghosted(c, p) for c in z["E"]]
+    B = [_ghosted(c, p) for c in z["B"]]
+    J = [_ghosted(c, p) for c in z["J"]]
+    Bh = O.update_B(E, B, p, do_filter=False)
+    E1 = O.update_E(E, Bh, J, p)
+    B1 = O.update_B(E1, Bh, p, do_filter=True)
+    lhs = sum(np.sum(lam["E"][c] * _interior(E1[c], p)) + np.sum(lam["B"][c] * _interior(B1[c], p))
+              for c in range(3))
+    dev = "cuda"
+    lE = torch.tensor(np.stack(lam["E"]), dtype=torch.float64, device=dev)
+    lB = torch.tensor(np.stack(lam["B"]), dtype=torch.float64, device=dev)
+    lJ = torch.empty_like(lE)
+    work = torch.empty((6,) + N_, dtype=torch.float64, device=dev)
+    w6 = (N.c_vp * 6)(*[work[c].data_ptr() for c in range(6)])
+    N.call("pic_adjoint_yee", desc, _p3(lE), _p3(lB), _p3(lJ), w6, N.stream_handle())
+    torch.cuda.synchronize()
+    aE, aB, aJ = lE.cpu().numpy(), lB.cpu().numpy(), lJ.cpu().numpy()
+    rhs = sum(np.sum(aE[c] * z["E"][c]) + np.sum(aB[c] * z["B"][c]) + np.sum(aJ[c] * z["J"][c]) for c in range(3))
+    assert abs(lhs - rhs) <= 1e-11 * (abs(lhs) + np.sqrt(sum(np.sum(a * a) for a in (aE, aB, aJ))))
+
+
+@pytest.mark.parametrize("N_", [(8, 6, 5), (9, 7, 1), (10, 1, 1)])
+def test_periodic_expand_is_refresh_and_fold_is_its_transpose(N_):
+    static, dyn = workloads.make_parameters(N_, dt=0.05)
+    p = oracle_params(static, dyn)
+    desc = make_desc(static, dyn)
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal(N_)
+    G = rng.standard_normal(p.shape)
+    x = torch.tensor(X, dtype=torch.float64, device="cuda")
+    g = torch.tensor(G, dtype=torch.float64, device="cuda")
+    xg = torch.empty_like(g)
+    acc = torch.zeros_like(x)
+    N.call("pic_periodic_expand_f64", desc, N.ptr(x), N.ptr(xg), N.stream_handle())
+    N.call("pic_periodic_fold_add_f64", desc, N.ptr(g), N.ptr(acc), N.stream_handle())
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(xg.cpu().numpy(), _ghosted(X, p))
+    np.testing.assert_allclose(acc.cpu().numpy(), _interior(O.fold(G, p), p), rtol=1e-13, atol=1e-13)
+    assert abs(np.sum(acc.cpu().numpy() * X) - np.sum(G * xg.cpu().numpy())) <= 1e-11 * np.sum(np.abs(G))
+
+
+# ------------------------------------------------------------ gradients
+
+def _setup(N_=(12, 10, 1), ppc=2, pusher="boris", sf=1):
+    static, dyn = workloads.make_parameters(N_, cfl=0.5, particle_pusher=pusher, shape_factor=sf)
+    w = workloads.plasma_frequency_weight(2 * ppc, dyn.dt, 0.3)
+    from pypic3d_amd.particles.particle_class import SpeciesConfig
+    cfg = SpeciesConfig(charge=np.array([-1.0, -1.0]), mass=np.array([1.0, 1.0]), weight=np.array([w, w]),
+                        update_x=np.ones((2, 3), bool), update_u=np.ones((2, 3), bool))
+    return static, dyn, cfg
+
+
+def _oracle_loss(x0, v0, p, sp, steps):
+    x = np.ascontiguousarray(np.moveaxis(x0, 0, -1))
+    u = np.ascontiguousarray(np.moveaxis(v0, 0, -1))
+    z = [np.zeros(p.shape) for _ in range(3)]
+    st = O.State(x, u, np.ones(x.shape[:2], bool), z, [c.copy() for c in z])
+    for _ in range(steps):
+        st = O.time_loop_electrodynamic(st, sp, p)
+    uE, uB, _ = O.compute_energy(st.u, st.active, sp, st.E, st.B, p)
+    return uE + uB
+
+
+def _gpu_grads(static, dyn, cfg, ppc, drift, amp, steps, k, seed=5):
+    d_ = torch.tensor(drift, dtype=torch.float64, requires_grad=True)
+    a_ = torch.tensor(amp, dtype=torch.float64, requires_grad=True)
+    x0, v0 = two_stream_state(static, dyn, ppc, d_, a_, vth=0.01, seed=seed, device="cpu")
+    x0c, v0c = x0.cuda(), v0.cuda()
+    g = int(static.guard_cells)
+    shape = (3, int(dyn.Nx) + 2 * g, int(dyn.Ny) + 2 * g, int(dyn.Nz) + 2 * g)
+    E0 = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    B0 = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    act = torch.ones((2, x0.shape[2]), dtype=torch.bool, device="cuda")
+    run = AdjointRun(static, dyn, cfg, act, checkpoint_every=k)
+    loss = field_energy_loss(x0c, v0c, E0, B0, run, steps)
+    gd, ga, gv = torch.autograd.grad(loss, [d_, a_, v0c])
+    return float(loss), float(gd), float(ga), gv.cpu().numpy()
+
+
+@pytest.mark.parametrize("pusher,sf", [("boris", 1), ("higuera_cary", 1), ("boris", 2)])
+def test_field_energy_gradients_match_oracle_central_differences(pusher, sf):
+    static, dyn, cfg = _setup(pusher=pusher, sf=sf)
+    p = oracle_params(static, dyn)
+    sp = oracle_species(cfg)
+    ppc, drift, amp, steps, h = 2, 0.1, 0.05, 12, 1e-4
+    loss, gd, ga, gv = _gpu_grads(static, dyn, cfg, ppc, drift, amp, steps, k=5)
+
+    def L(dr, am, dv=None):
+        x0, v0 = two_stream_state(static, dyn, ppc, dr, am, vth=0.01, seed=5, device="cpu")
+        v0 = v0.numpy().copy()
+        if dv is not None:
+            v0[dv[0]] += dv[1]
+        return _oracle_loss(x0.numpy(), v0, p, sp, steps)
+
+    l0 = L(drift, amp)
+    assert abs(loss - l0) <= 1e-4 * abs(l0)
+    fd_d = (L(drift + h, amp) - L(drift - h, amp)) / (2 * h)
+    fd_a = (L(drift, amp + h) - L(drift, amp - h)) / (2 * h)
+    assert abs(gd - fd_d) <= 1e-4 * abs(fd_d), (gd, fd_d)
+    assert abs(ga - fd_a) <= 1e-4 * abs(fd_a), (ga, fd_a)
+    scale = np.max(np.abs(gv))
+    rng = np.random.default_rng(1)
+    for _ in range(4):
+        idx = (int(rng.integers(3)), int(rng.integers(2)), int(rng.integers(gv.shape[2])))
+        fd = (L(drift, amp, (idx, h)) - L(drift, amp, (idx, -h))) / (2 * h)
+        assert abs(gv[idx] - fd) <= 1e-4 * abs(fd) + 1e-4 * scale, (idx, gv[idx], fd)
+
+
+def test_gradients_do_not_depend_on_the_checkpoint_interval():
+    static, dyn, cfg = _setup(N_=(10, 8, 1))
+    ref = _gpu_grads(static, dyn, cfg, 2, 0.1, 0.05, 9, k=9)
+    for k in (1, 4):
+        got = _gpu_grads(static, dyn, cfg, 2, 0.1, 0.05, 9, k=k)
+        assert abs(got[1] - ref[1]) <= 1e-6 * abs(ref[1])
+        assert abs(got[2] - ref[2]) <= 1e-6 * abs(ref[2])
+        assert np.max(np.abs(got[3] - ref[3])) <= 1e-6 * np.max(np.abs(ref[3]))
