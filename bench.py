@@ -106,6 +106,119 @@ def extras(n, ppc, steps, seed):
     return out
 
 
+def ensemble(n_members, steps, seed):
+    """Batched two-stream ensemble against sequential single-simulation runs
+    of the same members (both advanced from identical initial states)."""
+    import torch
+    from pypic3d_amd import workloads
+    from pypic3d_amd.batch import EnsembleRunner
+    from pypic3d_amd.runner import ElectrodynamicRunner
+
+    members = workloads.config_two_stream_ensemble(n_members=n_members, seed=seed)
+    batch = EnsembleRunner(members)
+    npart = batch.particles
+    (rounds, grid), tune = batch.autotune()
+    batch.capture()
+    reps = max(1, steps // 2)
+    batch.replay()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        batch.replay()
+    b.record()
+    torch.cuda.synchronize()
+    t_batch = a.elapsed_time(b) / (2 * reps)
+    del batch
+    torch.cuda.empty_cache()
+    singles = [ElectrodynamicRunner(*m) for m in members]
+    del members
+    for r in singles:
+        r.step()
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(steps):
+        for r in singles:
+            r.step()
+    b.record()
+    torch.cuda.synchronize()
+    t_seq = a.elapsed_time(b) / steps
+    # the same sequential launches recorded in one graph (no host launch cost)
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for r in singles:
+            r.step()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        for r in singles:       # two steps per member: the replay advances the state
+            r.step()
+            r.step()
+    g.replay()
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(reps):
+        g.replay()
+    b.record()
+    torch.cuda.synchronize()
+    t_seq_graph = a.elapsed_time(b) / (2 * reps)
+    return {"members": n_members, "particles": npart, "rounds": rounds, "grid": grid,
+            "batched_ms_per_step": t_batch, "sequential_ms_per_step": t_seq,
+            "sequential_graph_ms_per_step": t_seq_graph,
+            "speedup_vs_sequential": t_seq / t_batch, "speedup_vs_sequential_graph": t_seq_graph / t_batch,
+            "batched_particle_steps_per_s": npart / (t_batch * 1e-3)}
+
+
+def adjoint(n=64, ppc=32, steps=200, every=20, seed=20261202):
+    """Differentiable two-stream (n x n x 1, two beams x ppc): forward with a
+    checkpoint every `every` steps, then the adjoint sweep (recompute + reverse)
+    for d(U_E + U_B)/d(drift, amplitude, per-particle momenta)."""
+    import torch
+    from pypic3d_amd import workloads
+    from pypic3d_amd.adjoint import AdjointRun, field_energy_loss, two_stream_state
+    from pypic3d_amd.particles.particle_class import SpeciesConfig
+    import numpy as np
+
+    static, dyn = workloads.make_parameters((n, n, 1), cfl=0.5)
+    w = workloads.plasma_frequency_weight(2 * ppc, dyn.dt, 0.1)
+    cfg = SpeciesConfig(charge=np.array([-1.0, -1.0]), mass=np.array([1.0, 1.0]), weight=np.array([w, w]),
+                        update_x=np.ones((2, 3), bool), update_u=np.ones((2, 3), bool))
+    drift = torch.tensor(0.1, dtype=torch.float64, requires_grad=True)
+    amp = torch.tensor(1e-3, dtype=torch.float64, requires_grad=True)
+    x0, v0 = two_stream_state(static, dyn, ppc, drift, amp, vth=0.01, seed=seed, device="cuda")
+    g = int(static.guard_cells)
+    shape = (3, n + 2 * g, n + 2 * g, 1 + 2 * g)
+    E0 = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    B0 = torch.zeros_like(E0)
+    act = torch.ones((2, x0.shape[2]), dtype=torch.bool, device="cuda")
+    run = AdjointRun(static, dyn, cfg, act, checkpoint_every=every)
+    # plain forward cost: the same steps without checkpoints
+    run.forward(x0.detach(), v0.detach(), tuple(E0), tuple(B0), 2)
+    r = run.runner
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.step()
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t0) / steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = field_energy_loss(x0, v0, E0, B0, run, steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    gd, ga, gv = torch.autograd.grad(loss, [drift, amp, v0])
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return {"grid": [n, n, 1], "particles": int(x0.shape[1] * x0.shape[2]), "steps": steps,
+            "checkpoint_every": every, "forward_ms_per_step": 1e3 * fwd,
+            "forward_with_checkpoints_ms_per_step": 1e3 * (t1 - t0) / steps,
+            "adjoint_ms_per_step": 1e3 * (t2 - t1) / steps, "adjoint_over_forward": (t2 - t1) / steps / fwd,
+            "loss": float(loss), "dloss_ddrift": float(gd), "dloss_damplitude": float(ga),
+            "max_abs_dloss_dv0": float(gv.abs().max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +229,9 @@ def main():
     ap.add_argument("--sort-every", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble comparison")
+    ap.add_argument("--ensemble-members", type=int, default=256)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the J_from_rhov / rho / CG measurements of the same workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
@@ -211,6 +327,14 @@ def main():
         torch.cuda.empty_cache()
         print("[bench] J_from_rhov, rho and CG on the same workload", file=sys.stderr, flush=True)
         out["other_paths"] = extras(args.n, args.ppc, 3, 20261202)
+    if not args.no_ensemble:
+        print("[bench] batched ensemble vs sequential single-simulation launches", file=sys.stderr, flush=True)
+        out["ensemble"] = ensemble(args.ensemble_members, 20, 20261202)
+        torch.cuda.empty_cache()
+    if not args.no_adjoint:
+        print("[bench] adjoint step cost (differentiable two-stream)", file=sys.stderr, flush=True)
+        out["adjoint"] = adjoint()
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (NumPy oracle sample)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()

@@ -173,6 +173,36 @@ int pic_energy(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in
                const float* const E[3], const float* const B[3], double* out,
                void* workspace, int64_t workspace_bytes, void* stream);
 
+/* ---- batched ensembles ----------------------------------------------------
+ * Independent simulations of uneven grid size and particle count advance
+ * together, each with time_loop_electrodynamic's Esirkepov-branch semantics
+ * (evolve.py:16-103).  Particle planes and field components of all members
+ * are slices of shared batch arrays in the single-simulation layout.  The
+ * reference runs one simulation per process; this has no direct counterpart
+ * there. */
+typedef struct PicBatchSim {
+    PicDesc desc;
+    PicSpecies species;
+    int64_t particle_offset;   /* first slot of this member in the batch planes   */
+    int64_t field_offset;      /* first element of this member's field arrays     */
+} PicBatchSim;
+
+/* Host-side validation of a table (periodic fields and particles, alpha = 1,
+ * slices inside the batch arrays). */
+int pic_batch_check(const PicBatchSim* sims, int32_t n_sims, int64_t particle_slots, int64_t field_elems);
+/* Fused particle phase of every member: `sims` and `work` are device arrays;
+ * work holds n_work (member, block) pairs covering each member's slots in
+ * blocks of 256 * rounds (rounds 2, 4 or 8); `grid` persistent blocks walk
+ * the list.  J (batch arrays) must be zero on entry.  out must not alias in. */
+int pic_batch_fused_push_esirkepov(const PicBatchSim* sims, const int32_t* work, int32_t n_work, int32_t rounds,
+                                   int32_t grid, const PicParticles* in, const float* const E[3],
+                                   const float* const B[3], PicParticles* out, float* const J[3], void* stream);
+/* Grid phase of every member, one block per member: J fold + refresh
+ * (particle codes), B_half = B - dt/2 curl E_in, E_out = E_in + dt (C^2 curl
+ * B_half - J/eps), B = B_half - dt/2 curl E_out, each refreshed. */
+int pic_batch_field_step(const PicBatchSim* sims, int32_t n_sims, const float* const E_in[3], float* const B[3],
+                         float* const B_half[3], float* const J[3], float* const E_out[3], void* stream);
+
 /* ---- adjoint of the electrodynamic Esirkepov step (periodic fields and
  * particles; the reference has no gradient code -- these are the reverse-mode
  * transposes of evolve.py:16-103 as computed by the entry points above).

@@ -83,6 +83,9 @@ SIGNATURES = {
     "pic_efield_from_phi": (c_int, [DP, c_vp, P3, c_vp]),
     "pic_energy_workspace_bytes": (c_i64, [DP, SP]),
     "pic_energy": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, c_i64, c_vp]),
+    "pic_batch_check": (c_int, [c_vp, c_i32, c_i64, c_i64]),
+    "pic_batch_fused_push_esirkepov": (c_int, [c_vp, c_vp, c_i32, c_i32, c_i32, PP, P3, P3, PP, P3, c_vp]),
+    "pic_batch_field_step": (c_int, [c_vp, c_i32, P3, P3, P3, P3, P3, c_vp]),
     "pic_fill_f64": (c_int, [c_vp, c_i64, c_dbl, c_vp]),
     "pic_periodic_expand_f64": (c_int, [DP, c_vp, c_vp, c_vp]),
     "pic_periodic_fold_add_f64": (c_int, [DP, c_vp, c_vp, c_vp]),

@@ -1,0 +1,170 @@
+// Per-element field operations shared by the one-launch-per-op kernels
+// (pic_fields.hip) and the batched one-block-per-simulation field step
+// (pic_batch.hip).  Each function handles one element id `t` of its index
+// space; callers map threads to ids.
+#pragma once
+#include "pic_common.h"
+
+namespace pic {
+
+__device__ inline int64_t slab_to_index(const PicDesc& d, int axis, int pos, int a, int b) {
+    if (axis == 0) return fidx(d, pos, a, b);
+    if (axis == 1) return fidx(d, a, pos, b);
+    return fidx(d, a, b, pos);
+}
+
+__host__ __device__ inline void other_dims(const PicDesc& d, int axis, int& da, int& db) {
+    da = d.dims[axis == 0 ? 1 : 0];
+    db = d.dims[axis == 2 ? 1 : 2];
+}
+
+// Number of element ids of each operation on one axis.
+__host__ __device__ inline int64_t refresh_elems(const PicDesc& d, int axis) {
+    int da, db;
+    other_dims(d, axis, da, db);
+    return (int64_t)2 * d.g * da * db;
+}
+__host__ __device__ inline int64_t fold_elems(const PicDesc& d, int axis) {
+    int da, db;
+    other_dims(d, axis, da, db);
+    return d.n[axis] == 1 ? (int64_t)da * db : (int64_t)d.g * da * db;
+}
+
+// ghost_cells.py:142-196: ghost element t (< refresh_elems) of one axis.
+__device__ inline void refresh_elem(const PicDesc& d, float* f, int axis, int mode, int64_t t) {
+    int g = d.g, n = d.n[axis];
+    int da, db;
+    other_dims(d, axis, da, db);
+    int b = (int)(t % db);
+    int64_t r = t / db;
+    int a = (int)(r % da);
+    int l = (int)(r / da);
+    int dst = l < g ? l : n + l;          // upper ghosts n+g .. n+2g-1
+    float v = 0.0f;
+    if (mode == MODE_REDUCED_COPY) {
+        v = f[slab_to_index(d, axis, g, a, b)];
+    } else if (mode == MODE_PERIODIC) {
+        int src = l < g ? n + l : l;      // lower l <- n+l ; upper (l-g) <- g+(l-g)
+        v = f[slab_to_index(d, axis, src, a, b)];
+    }
+    f[slab_to_index(d, axis, dst, a, b)] = v;
+}
+
+// ghost_cells.py:218-289: fold element t (< fold_elems) of a non-reduced axis.
+// bc: 0 periodic self-exchange, 1 conducting reflection, other: dropped.
+__device__ inline void fold_elem(const PicDesc& d, float* f, int axis, int bc, int64_t t) {
+    int g = d.g, n = d.n[axis];
+    int da, db;
+    other_dims(d, axis, da, db);
+    int b = (int)(t % db);
+    int64_t r = t / db;
+    int a = (int)(r % da);
+    int l = (int)(r / da);
+    int64_t lo_i = slab_to_index(d, axis, l, a, b);
+    int64_t hi_i = slab_to_index(d, axis, n + g + l, a, b);
+    float lo = f[lo_i];
+    float hi = f[hi_i];
+    f[lo_i] = 0.0f;
+    f[hi_i] = 0.0f;
+    if (bc == 0) {
+        atomic_add_f32(&f[slab_to_index(d, axis, n + l, a, b)], lo);
+        atomic_add_f32(&f[slab_to_index(d, axis, g + l, a, b)], hi);
+    } else if (bc == 1) {
+        atomic_add_f32(&f[slab_to_index(d, axis, g + l, a, b)], -lo);
+        atomic_add_f32(&f[slab_to_index(d, axis, n + l, a, b)], -hi);
+    }
+}
+
+// ghost_cells.py:218-235: fold element t of a reduced (single-cell) axis.
+__device__ inline void fold_reduced_elem(const PicDesc& d, float* f, int axis, int bc, int64_t t) {
+    int g = d.g;
+    int da, db;
+    other_dims(d, axis, da, db);
+    int b = (int)(t % db);
+    int a = (int)(t / db);
+    float lower = 0.0f, upper = 0.0f;
+    for (int l = 0; l < g; ++l) {
+        int64_t i0 = slab_to_index(d, axis, l, a, b);
+        int64_t i1 = slab_to_index(d, axis, 1 + g + l, a, b);
+        lower += f[i0];
+        upper += f[i1];
+        f[i0] = 0.0f;
+        f[i1] = 0.0f;
+    }
+    float s = lower + upper;
+    int64_t c = slab_to_index(d, axis, g, a, b);
+    if (bc == 0) f[c] += s;
+    else if (bc == 1) f[c] += -s;
+}
+
+// Value of a refreshed copy of f at (i,j,k) where exactly one coordinate may
+// sit one cell outside the interior along `axis`.
+__device__ inline float refreshed_at(const float* __restrict__ f, const PicDesc& d, const int mode[3], int i, int j,
+                                     int k, int axis) {
+    int c[3] = {i, j, k};
+    int src = ghost_source(c[axis], d.n[axis], d.g, mode[axis]);
+    if (src < 0) return 0.0f;
+    c[axis] = src;
+    return f[fidx(d, c[0], c[1], c[2])];
+}
+
+struct YeeConst {
+    float hdt, dt, c2, inv_eps, id[3];
+    int mode[3];
+};
+
+__host__ __device__ inline YeeConst yee_const(const PicDesc& d) {
+    YeeConst y;
+    y.hdt = (float)(d.dt / 2);
+    y.dt = (float)d.dt;
+    y.c2 = (float)(d.C * d.C);
+    y.inv_eps = (float)(1.0 / d.eps);
+    for (int a = 0; a < 3; ++a) {
+        y.id[a] = (float)(1.0 / d.d[a]);
+        y.mode[a] = refresh_mode(d, a, 0);
+    }
+    return y;
+}
+
+// first_order_yee.py:96-142 at interior array point (i, j, k):
+// B_out = B_in - (dt/2) curl E with forward differences.
+__device__ inline void yee_b_point(const PicDesc& d, const YeeConst& y, const float* const E[3],
+                                   const float* const B[3], float* const out[3], int i, int j, int k) {
+    int64_t c = fidx(d, i, j, k);
+    float ex = E[0][c], ey = E[1][c], ez = E[2][c];
+    float ez_yp = refreshed_at(E[2], d, y.mode, i, j + 1, k, 1);
+    float ey_zp = refreshed_at(E[1], d, y.mode, i, j, k + 1, 2);
+    float ex_zp = refreshed_at(E[0], d, y.mode, i, j, k + 1, 2);
+    float ez_xp = refreshed_at(E[2], d, y.mode, i + 1, j, k, 0);
+    float ey_xp = refreshed_at(E[1], d, y.mode, i + 1, j, k, 0);
+    float ex_yp = refreshed_at(E[0], d, y.mode, i, j + 1, k, 1);
+    float curl_x = (ez_yp - ez) * y.id[1] - (ey_zp - ey) * y.id[2];
+    float curl_y = (ex_zp - ex) * y.id[2] - (ez_xp - ez) * y.id[0];
+    float curl_z = (ey_xp - ey) * y.id[0] - (ex_yp - ex) * y.id[1];
+    out[0][c] = B[0][c] - y.hdt * curl_x;
+    out[1][c] = B[1][c] - y.hdt * curl_y;
+    out[2][c] = B[2][c] - y.hdt * curl_z;
+}
+
+// first_order_yee.py:12-72 at interior array point (i, j, k):
+// E_out = E_in + dt (C^2 curl B - J/eps) with backward differences.
+__device__ inline void yee_e_point(const PicDesc& d, const YeeConst& y, const float* const E[3],
+                                   const float* const B[3], const float* const J[3], float* const out[3], int i,
+                                   int j, int k) {
+    int64_t c = fidx(d, i, j, k);
+    float bx = B[0][c], by = B[1][c], bz = B[2][c];
+    float bz_ym = refreshed_at(B[2], d, y.mode, i, j - 1, k, 1);
+    float by_zm = refreshed_at(B[1], d, y.mode, i, j, k - 1, 2);
+    float bx_zm = refreshed_at(B[0], d, y.mode, i, j, k - 1, 2);
+    float bz_xm = refreshed_at(B[2], d, y.mode, i - 1, j, k, 0);
+    float by_xm = refreshed_at(B[1], d, y.mode, i - 1, j, k, 0);
+    float bx_ym = refreshed_at(B[0], d, y.mode, i, j - 1, k, 1);
+    float curl_x = (bz - bz_ym) * y.id[1] - (by - by_zm) * y.id[2];
+    float curl_y = (bx - bx_zm) * y.id[2] - (bz - bz_xm) * y.id[0];
+    float curl_z = (by - by_xm) * y.id[0] - (bx - bx_ym) * y.id[1];
+    out[0][c] = E[0][c] + (y.c2 * curl_x - J[0][c] * y.inv_eps) * y.dt;
+    out[1][c] = E[1][c] + (y.c2 * curl_y - J[1][c] * y.inv_eps) * y.dt;
+    out[2][c] = E[2][c] + (y.c2 * curl_z - J[2][c] * y.inv_eps) * y.dt;
+}
+
+}  // namespace pic

@@ -5,9 +5,11 @@
 // values a freshly refreshed copy of the input would hold
 // (first_order_yee.py:31, :115 refresh before differencing), so the caller's
 // input arrays are never modified -- the reference is functional.
-#include "pic_common.h"
+#include "pic_field_ops.h"
 
 namespace {
+
+using namespace pic;
 
 constexpr int kBlock = 256;
 
@@ -17,100 +19,29 @@ __global__ void fill_kernel(float* __restrict__ p, int64_t n, float v) {
     for (; i < n; i += stride) p[i] = v;
 }
 
-// Decompose a ghost-slab element id along `axis` into (slab index l, a, b),
-// with (a, b) the other two coordinates in increasing axis order.
-struct SlabCoord {
-    int l, a, b;
-};
-
-__device__ inline int64_t slab_to_index(const PicDesc& d, int axis, int pos, int a, int b) {
-    if (axis == 0) return fidx(d, pos, a, b);
-    if (axis == 1) return fidx(d, a, pos, b);
-    return fidx(d, a, b, pos);
-}
-
-__device__ inline void other_dims(const PicDesc& d, int axis, int& da, int& db) {
-    da = d.dims[axis == 0 ? 1 : 0];
-    db = d.dims[axis == 2 ? 1 : 2];
-}
-
 // ghost_cells.py:142-196: one axis of the refresh, for up to three fields.
 __global__ void refresh_axis_kernel(PicDesc d, float* f0, float* f1, float* f2, int axis, int mode) {
-    int g = d.g, n = d.n[axis];
-    int da, db;
-    other_dims(d, axis, da, db);
-    int64_t per = (int64_t)2 * g * da * db;
+    const int64_t per = refresh_elems(d, axis);
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= per * 3) return;
     int which = (int)(t / per);
-    int64_t r = t - which * per;
     float* f = which == 0 ? f0 : (which == 1 ? f1 : f2);
     if (f == nullptr) return;
-    int b = (int)(r % db);
-    r /= db;
-    int a = (int)(r % da);
-    int l = (int)(r / da);
-    int dst = l < g ? l : n + l;          // upper ghosts n+g .. n+2g-1
-    float v = 0.0f;
-    if (mode == MODE_REDUCED_COPY) {
-        v = f[slab_to_index(d, axis, g, a, b)];
-    } else if (mode == MODE_PERIODIC) {
-        int src = l < g ? n + l : l;      // lower l <- n+l ; upper (l-g) <- g+(l-g)
-        v = f[slab_to_index(d, axis, src, a, b)];
-    }
-    f[slab_to_index(d, axis, dst, a, b)] = v;
+    refresh_elem(d, f, axis, mode, t - which * per);
 }
 
 // ghost_cells.py:218-289: one axis of the fold (non-reduced axis).
-// bc: 0 periodic self-exchange, 1 conducting reflection, other: dropped.
 __global__ void fold_axis_kernel(PicDesc d, float* f, int axis, int bc) {
-    int g = d.g, n = d.n[axis];
-    int da, db;
-    other_dims(d, axis, da, db);
-    int64_t per = (int64_t)g * da * db;
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= per) return;
-    int b = (int)(t % db);
-    int64_t r = t / db;
-    int a = (int)(r % da);
-    int l = (int)(r / da);
-    int64_t lo_i = slab_to_index(d, axis, l, a, b);
-    int64_t hi_i = slab_to_index(d, axis, n + g + l, a, b);
-    float lo = f[lo_i];
-    float hi = f[hi_i];
-    f[lo_i] = 0.0f;
-    f[hi_i] = 0.0f;
-    if (bc == 0) {
-        atomic_add_f32(&f[slab_to_index(d, axis, n + l, a, b)], lo);
-        atomic_add_f32(&f[slab_to_index(d, axis, g + l, a, b)], hi);
-    } else if (bc == 1) {
-        atomic_add_f32(&f[slab_to_index(d, axis, g + l, a, b)], -lo);
-        atomic_add_f32(&f[slab_to_index(d, axis, n + l, a, b)], -hi);
-    }
+    if (t >= fold_elems(d, axis)) return;
+    fold_elem(d, f, axis, bc, t);
 }
 
 // ghost_cells.py:218-235: fold of a reduced (single-cell) axis.
 __global__ void fold_reduced_kernel(PicDesc d, float* f, int axis, int bc) {
-    int g = d.g;
-    int da, db;
-    other_dims(d, axis, da, db);
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)da * db) return;
-    int b = (int)(t % db);
-    int a = (int)(t / db);
-    float lower = 0.0f, upper = 0.0f;
-    for (int l = 0; l < g; ++l) {
-        int64_t i0 = slab_to_index(d, axis, l, a, b);
-        int64_t i1 = slab_to_index(d, axis, 1 + g + l, a, b);
-        lower += f[i0];
-        upper += f[i1];
-        f[i0] = 0.0f;
-        f[i1] = 0.0f;
-    }
-    float s = lower + upper;
-    int64_t c = slab_to_index(d, axis, g, a, b);
-    if (bc == 0) f[c] += s;
-    else if (bc == 1) f[c] += -s;
+    if (t >= fold_elems(d, axis)) return;
+    fold_reduced_elem(d, f, axis, bc, t);
 }
 
 // Zero the wall planes g and n+g-1 of one axis (ghost_cells.py:344-362).
@@ -176,77 +107,25 @@ __global__ void filter_kernel(PicDesc d, const float* __restrict__ in, float* __
     }
 }
 
-// Value of a refreshed copy of f at (i,j,k) where exactly one coordinate may
-// sit one cell outside the interior along `axis`.
-__device__ inline float refreshed_at(const float* __restrict__ f, const PicDesc& d, const int mode[3], int i,
-                                     int j, int k, int axis) {
-    int c[3] = {i, j, k};
-    int src = ghost_source(c[axis], d.n[axis], d.g, mode[axis]);
-    if (src < 0) return 0.0f;
-    c[axis] = src;
-    return f[fidx(d, c[0], c[1], c[2])];
-}
+struct F3 {
+    const float* p[3];
+};
+struct W3f {
+    float* p[3];
+};
 
 // first_order_yee.py:96-142: B_out = B_in - (dt/2) curl E (forward diffs).
-__global__ void yee_b_kernel(PicDesc d, const float* __restrict__ Ex, const float* __restrict__ Ey,
-                             const float* __restrict__ Ez, const float* __restrict__ Bx,
-                             const float* __restrict__ By, const float* __restrict__ Bz, float* __restrict__ ox,
-                             float* __restrict__ oy, float* __restrict__ oz, float hdt, float idx_, float idy,
-                             float idz, int m0, int m1, int m2) {
+__global__ void yee_b_kernel(PicDesc d, YeeConst y, F3 E, F3 B, W3f out) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y;
-    int i = blockIdx.z;
     if (k >= d.n[2]) return;
-    i += d.g;
-    j += d.g;
-    k += d.g;
-    const int mode[3] = {m0, m1, m2};
-    int64_t c = fidx(d, i, j, k);
-    float ex = Ex[c], ey = Ey[c], ez = Ez[c];
-    float ez_yp = refreshed_at(Ez, d, mode, i, j + 1, k, 1);
-    float ey_zp = refreshed_at(Ey, d, mode, i, j, k + 1, 2);
-    float ex_zp = refreshed_at(Ex, d, mode, i, j, k + 1, 2);
-    float ez_xp = refreshed_at(Ez, d, mode, i + 1, j, k, 0);
-    float ey_xp = refreshed_at(Ey, d, mode, i + 1, j, k, 0);
-    float ex_yp = refreshed_at(Ex, d, mode, i, j + 1, k, 1);
-    float curl_x = (ez_yp - ez) * idy - (ey_zp - ey) * idz;
-    float curl_y = (ex_zp - ex) * idz - (ez_xp - ez) * idx_;
-    float curl_z = (ey_xp - ey) * idx_ - (ex_yp - ex) * idy;
-    ox[c] = Bx[c] - hdt * curl_x;
-    oy[c] = By[c] - hdt * curl_y;
-    oz[c] = Bz[c] - hdt * curl_z;
+    yee_b_point(d, y, E.p, B.p, out.p, blockIdx.z + d.g, blockIdx.y + d.g, k + d.g);
 }
 
 // first_order_yee.py:12-72: E_out = E_in + dt (C^2 curl B - J/eps) (backward diffs).
-__global__ void yee_e_kernel(PicDesc d, const float* __restrict__ Ex, const float* __restrict__ Ey,
-                             const float* __restrict__ Ez, const float* __restrict__ Bx,
-                             const float* __restrict__ By, const float* __restrict__ Bz,
-                             const float* __restrict__ Jx, const float* __restrict__ Jy,
-                             const float* __restrict__ Jz, float* __restrict__ ox, float* __restrict__ oy,
-                             float* __restrict__ oz, float dt, float c2, float inv_eps, float idx_, float idy,
-                             float idz, int m0, int m1, int m2) {
+__global__ void yee_e_kernel(PicDesc d, YeeConst y, F3 E, F3 B, F3 J, W3f out) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y;
-    int i = blockIdx.z;
     if (k >= d.n[2]) return;
-    i += d.g;
-    j += d.g;
-    k += d.g;
-    const int mode[3] = {m0, m1, m2};
-    int64_t c = fidx(d, i, j, k);
-    float bx = Bx[c], by = By[c], bz = Bz[c];
-    float bz_ym = refreshed_at(Bz, d, mode, i, j - 1, k, 1);
-    float by_zm = refreshed_at(By, d, mode, i, j, k - 1, 2);
-    float bx_zm = refreshed_at(Bx, d, mode, i, j, k - 1, 2);
-    float bz_xm = refreshed_at(Bz, d, mode, i - 1, j, k, 0);
-    float by_xm = refreshed_at(By, d, mode, i - 1, j, k, 0);
-    float bx_ym = refreshed_at(Bx, d, mode, i, j - 1, k, 1);
-    float curl_x = (bz - bz_ym) * idy - (by - by_zm) * idz;
-    float curl_y = (bx - bx_zm) * idz - (bz - bz_xm) * idx_;
-    float curl_z = (by - by_xm) * idx_ - (bx - bx_ym) * idy;
-    ox[c] = Ex[c] + (c2 * curl_x - Jx[c] * inv_eps) * dt;
-    oy[c] = Ey[c] + (c2 * curl_y - Jy[c] * inv_eps) * dt;
-    oz[c] = Ez[c] + (c2 * curl_z - Jz[c] * inv_eps) * dt;
+    yee_e_point(d, y, E.p, B.p, J.p, out.p, blockIdx.z + d.g, blockIdx.y + d.g, k + d.g);
 }
 
 // electrostatic_yee.py:212-246: E interior = -(phi[+1]-phi[-1])/(2d), reading
@@ -388,10 +267,8 @@ int pic_yee_update_b(const PicDesc* desc, const float* const E[3], const float* 
         for (int e = 0; e < 3; ++e) PIC_REQUIRE((const float*)B_out[c] != E[e]);
     hipStream_t s = as_stream(stream);
     int block = interior_block(desc);
-    hipLaunchKernelGGL(yee_b_kernel, interior_grid(desc, block), dim3(block), 0, s, *desc, E[0], E[1], E[2],
-                       B_in[0], B_in[1], B_in[2], B_out[0], B_out[1], B_out[2], (float)(desc->dt / 2),
-                       (float)(1.0 / desc->d[0]), (float)(1.0 / desc->d[1]), (float)(1.0 / desc->d[2]),
-                       refresh_mode(*desc, 0, 0), refresh_mode(*desc, 1, 0), refresh_mode(*desc, 2, 0));
+    hipLaunchKernelGGL(yee_b_kernel, interior_grid(desc, block), dim3(block), 0, s, *desc, yee_const(*desc),
+                       F3{{E[0], E[1], E[2]}}, F3{{B_in[0], B_in[1], B_in[2]}}, W3f{{B_out[0], B_out[1], B_out[2]}});
     PIC_CHECK_LAUNCH();
     return pic_refresh_fields(desc, B_out[0], B_out[1], B_out[2], 0, s);
 }
@@ -404,11 +281,9 @@ int pic_yee_update_e(const PicDesc* desc, const float* const E_in[3], const floa
         for (int e = 0; e < 3; ++e) PIC_REQUIRE((const float*)E_out[c] != B[e] && (const float*)E_out[c] != J[e]);
     hipStream_t s = as_stream(stream);
     int block = interior_block(desc);
-    hipLaunchKernelGGL(yee_e_kernel, interior_grid(desc, block), dim3(block), 0, s, *desc, E_in[0], E_in[1], E_in[2],
-                       B[0], B[1], B[2], J[0], J[1], J[2], E_out[0], E_out[1], E_out[2], (float)desc->dt,
-                       (float)(desc->C * desc->C), (float)(1.0 / desc->eps), (float)(1.0 / desc->d[0]),
-                       (float)(1.0 / desc->d[1]), (float)(1.0 / desc->d[2]), refresh_mode(*desc, 0, 0),
-                       refresh_mode(*desc, 1, 0), refresh_mode(*desc, 2, 0));
+    hipLaunchKernelGGL(yee_e_kernel, interior_grid(desc, block), dim3(block), 0, s, *desc, yee_const(*desc),
+                       F3{{E_in[0], E_in[1], E_in[2]}}, F3{{B[0], B[1], B[2]}}, F3{{J[0], J[1], J[2]}},
+                       W3f{{E_out[0], E_out[1], E_out[2]}});
     PIC_CHECK_LAUNCH();
     return pic_refresh_fields(desc, E_out[0], E_out[1], E_out[2], 0, s);
 }

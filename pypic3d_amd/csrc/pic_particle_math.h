@@ -68,11 +68,13 @@ struct FieldPtrs {
 //   qw  = charge * weight                              (Esirkepov.py:102)
 //   h   = charge * dt / (2 mass)                       (boris.py:14-124)
 //   pre = -(qw / (d_o1 d_o2)) / dt per current axis    (Esirkepov.py:197-214)
+//   q_dv = qw / (dx dy dz), the factor of a reduced component (:197-214)
 //   ux, uu = update_x / update_u as bit masks          (particle_class.py:5-14)
 struct SpeciesK {
     double qw;
     float h;
     float pre[3];
+    float q_dv;
     uint32_t ux, uu;
 };
 
@@ -83,6 +85,7 @@ __device__ inline SpeciesK species_constants(const PicDesc& d, const PicSpecies&
     k.pre[0] = (float)(-(k.qw / (d.d[1] * d.d[2])) / d.dt);
     k.pre[1] = (float)(-(k.qw / (d.d[0] * d.d[2])) / d.dt);
     k.pre[2] = (float)(-(k.qw / (d.d[0] * d.d[1])) / d.dt);
+    k.q_dv = (float)(k.qw / (d.d[0] * d.d[1] * d.d[2]));
     k.ux = k.uu = 0u;
     for (int a = 0; a < 3; ++a) {
         k.ux |= (sp.update_x[s][a] ? 1u : 0u) << a;

@@ -177,3 +177,28 @@ def config_two_stream_2d(n=64, ppc=32):
     w = plasma_frequency_weight(2 * ppc, dyn.dt, 0.1)
     return static, dyn, [SpeciesSpec(ppc, weight=w, vth=(0.01,) * 3, drift=(0.1, 0, 0), perturb=1e-3),
                          SpeciesSpec(ppc, weight=w, vth=(0.01,) * 3, drift=(-0.1, 0, 0), perturb=1e-3)]
+
+
+def config_two_stream_ensemble(n_members=256, seed=20261202, device="cuda", ppc_range=(16, 128),
+                               nx_choices=(64, 128, 256, 512), ny_choices=(1, 32), drift_range=(0.05, 0.2),
+                               vth=0.01):
+    """Independent 1-D / 2-D two-stream members of uneven size: Nx from
+    nx_choices, Ny from ny_choices (Nz = 1), particles per cell per beam
+    uniform in ppc_range, drift uniform in drift_range, thermal spread vth,
+    dx = 1, cfl 0.5, omega_p dt = 0.1.  Returns a list of
+    (particles, species_config, fields, static, dynamic)."""
+    import random
+    rng = random.Random(int(seed))
+    members = []
+    for m in range(n_members):
+        nx = rng.choice(nx_choices)
+        ny = rng.choice(ny_choices)
+        ppc = rng.randint(*ppc_range)
+        drift = rng.uniform(*drift_range)
+        static, dyn = make_parameters((nx, ny, 1), cfl=0.5, name=f"member{m}")
+        w = plasma_frequency_weight(2 * ppc, dyn.dt, 0.1)
+        specs = [SpeciesSpec(ppc, weight=w, vth=(vth,) * 3, drift=(drift, 0, 0), perturb=1e-3),
+                 SpeciesSpec(ppc, weight=w, vth=(vth,) * 3, drift=(-drift, 0, 0), perturb=1e-3)]
+        parts, cfg = plasma(static, dyn, specs, seed=rng.randrange(1 << 30), device=device)
+        members.append((parts, cfg, empty_fields(static, dyn, device=device), static, dyn))
+    return members

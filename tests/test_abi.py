@@ -38,13 +38,16 @@ def test_struct_layout_matches_c(tmp_path):
     from pypic3d_amd import _native as N
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "pypic3d_amd.h"\n'
-                   'int main(){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(PicDesc), sizeof(PicSpecies),'
-                   ' sizeof(PicParticles), offsetof(PicDesc, d), offsetof(PicDesc, dt), offsetof(PicSpecies, update_u));}')
+                   'int main(){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(PicDesc), sizeof(PicSpecies),'
+                   ' sizeof(PicParticles), offsetof(PicDesc, d), offsetof(PicDesc, dt), offsetof(PicSpecies, update_u),'
+                   ' sizeof(PicBatchSim), offsetof(PicBatchSim, species), offsetof(PicBatchSim, field_offset));}')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    from pypic3d_amd.batch import PicBatchSim
     want = [ctypes.sizeof(N.PicDesc), ctypes.sizeof(N.PicSpecies), ctypes.sizeof(N.PicParticles),
-            N.PicDesc.d.offset, N.PicDesc.dt.offset, N.PicSpecies.update_u.offset]
+            N.PicDesc.d.offset, N.PicDesc.dt.offset, N.PicSpecies.update_u.offset,
+            ctypes.sizeof(PicBatchSim), PicBatchSim.species.offset, PicBatchSim.field_offset.offset]
     assert [int(v) for v in out] == want
 
 
