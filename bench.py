@@ -229,6 +229,8 @@ def main():
     ap.add_argument("--sort-every", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="use this fused-kernel launch shape instead of autotuning")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble comparison")
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
@@ -270,7 +272,11 @@ def main():
     for _ in range(args.warmup):
         runner.step()
     torch.cuda.synchronize()
-    variant, tune_ms = runner.autotune() if not args.no_autotune else (0, {})
+    if args.variant is not None:
+        runner.desc.variant = variant = args.variant
+        tune_ms = {}
+    else:
+        variant, tune_ms = runner.autotune() if not args.no_autotune else (0, {})
     print(f"[bench] autotune fused variants (ms): {tune_ms} -> {variant}", file=sys.stderr, flush=True)
     runner.step()
     torch.cuda.synchronize()

@@ -226,9 +226,10 @@ int pic_adjoint_yee(const PicDesc* desc, double* const lamE[3], double* const la
  * Esirkepov.py deposit, particle_tile_communication.py:82-99 position update)
  * at the step's start state `state` and fields E, B (+ external):
  * lam_x, lam_v in = adjoints of x', v'; out (in place) = adjoints of x, v.
- * lamJ: ghosted expansion of the adjoint of J; lamE, lamB: ghosted FP64
- * accumulators (zeroed by the caller) receiving the adjoint of the gathered
- * fields, to be folded with pic_periodic_fold_add_f64. */
+ * lamJ: interior adjoint of J (read at each deposit node's periodic partner,
+ * the transpose of fold + refresh); lamE, lamB: interior FP64 arrays to which
+ * the adjoint of the gathered fields is added (atomics at the gather nodes'
+ * periodic partners, the transpose of the ghost refresh). */
 int pic_adjoint_particles(const PicDesc* desc, const PicSpecies* sp, const PicParticles* state,
                           const float* const E[3], const float* const B[3], const float* const E_ext[3],
                           const float* const B_ext[3], const double* const lamJ[3], double* const lam_x[3],

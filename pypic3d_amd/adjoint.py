@@ -2,8 +2,8 @@
 
 The reference has no gradient code; this module differentiates the step it
 defines (evolve.py:16-103) with hand-written adjoint kernels
-(``pic_adjoint_yee``, ``pic_adjoint_particles`` and the periodic
-expand / fold transposes) behind ``torch.autograd.Function``:
+(``pic_adjoint_yee``, ``pic_adjoint_particles``) behind
+``torch.autograd.Function``:
 
 * ``AdjointRun`` advances the allocation-free runner from (x0, v0, E0, B0),
   keeping a checkpoint every ``checkpoint_every`` steps, and evaluates the
@@ -126,36 +126,37 @@ class AdjointRun:
         lamB = torch.stack([self._interior(c).double() * (dV / d.mu) for c in r.B]).contiguous()
         lamJ = torch.empty_like(lamE)
         work = torch.empty((6,) + tuple(lamE.shape[1:]), dtype=torch.float64, device=dev)
-        gshape = tuple(r.J[0].shape)
-        lamJg = torch.empty((3,) + gshape, dtype=torch.float64, device=dev)
-        lamEg = torch.empty_like(lamJg)
-        lamBg = torch.empty_like(lamJg)
         lx = torch.zeros((3, r.S, r.P), dtype=torch.float64, device=dev)
         lv = torch.zeros_like(lx)
         p3 = lambda t: N.P3(*[t[c].data_ptr() for c in range(3)])   # noqa: E731
         w6 = (N.c_vp * 6)(*[work[c].data_ptr() for c in range(6)])
+        LE, LB, LJ, LX, LV = p3(lamE), p3(lamB), p3(lamJ), p3(lx), p3(lv)
+        # one segment of recomputed states, allocated once
+        k = min(self.k, self.steps)
+        gshape = tuple(r.J[0].shape)
+        seg_x = torch.empty((k, 3, r.S, r.P), dtype=torch.float32, device=dev)
+        seg_v = torch.empty_like(seg_x)
+        seg_a = torch.empty((k, r.S, r.P), dtype=torch.bool, device=dev)
+        seg_E = torch.empty((k, 3) + gshape, dtype=torch.float32, device=dev)
+        seg_B = torch.empty_like(seg_E)
         starts = sorted(self.ckpt)
         for si in reversed(range(len(starts))):
             n0 = starts[si]
             n1 = starts[si + 1] if si + 1 < len(starts) else self.steps
             self._restore(self.ckpt[n0])
-            states = []
-            for _ in range(n0, n1):
-                states.append(self._snapshot())
+            for m in range(n1 - n0):
+                seg_x[m].copy_(r.pos[r.cur])
+                seg_v[m].copy_(r.vel[r.cur])
+                seg_a[m].copy_(r.act[r.cur])
+                torch.stack(r.E[r.ecur], out=seg_E[m])
+                torch.stack(r.B, out=seg_B[m])
                 r.step()
-            for st in reversed(states):
-                pos, vel, act, E, B = st
-                N.call("pic_adjoint_yee", d, p3(lamE), p3(lamB), p3(lamJ), w6, s)
-                for c in range(3):
-                    N.call("pic_periodic_expand_f64", d, N.ptr(lamJ[c]), N.ptr(lamJg[c]), s)
-                N.call("pic_fill_f64", N.ptr(lamEg), lamEg.numel(), 0.0, s)
-                N.call("pic_fill_f64", N.ptr(lamBg), lamBg.numel(), 0.0, s)
-                pp = pic_particles([pos[0], pos[1], pos[2]], [vel[0], vel[1], vel[2]], act)
-                N.call("pic_adjoint_particles", d, r.sp, pp, N.p3(E), N.p3(B), r.Ee, r.Be, p3(lamJg), p3(lx),
-                       p3(lv), p3(lamEg), p3(lamBg), s)
-                for c in range(3):
-                    N.call("pic_periodic_fold_add_f64", d, N.ptr(lamEg[c]), N.ptr(lamE[c]), s)
-                    N.call("pic_periodic_fold_add_f64", d, N.ptr(lamBg[c]), N.ptr(lamB[c]), s)
+            for m in reversed(range(n1 - n0)):
+                N.call("pic_adjoint_yee", d, LE, LB, LJ, w6, s)
+                pp = pic_particles([seg_x[m, 0], seg_x[m, 1], seg_x[m, 2]], [seg_v[m, 0], seg_v[m, 1], seg_v[m, 2]],
+                                   seg_a[m])
+                N.call("pic_adjoint_particles", d, r.sp, pp, p3(seg_E[m]), p3(seg_B[m]), r.Ee, r.Be, LJ, LX, LV, LE,
+                       LB, s)
         return lx, lv, lamE, lamB
 
 

@@ -15,13 +15,14 @@
 //  * Yee: on interior arrays with periodic wrap the transpose of curl_f is
 //    curl_b and vice versa, and the symmetric digital filter is self-adjoint.
 //  * fold + refresh of J transposes to "read the adjoint at the raw node's
-//    periodic partner": the adjoint of J is expanded to a ghosted array.
+//    periodic partner": the deposit stencil reads the interior adjoint of J
+//    at the partner of each (ghosted) node.
 //  * deposit -> gather: each particle reads the adjoint of J on its Esirkepov
 //    stencil and contracts it with dJ/d(x, v'), computed by forward-mode dual
 //    numbers through the same 5-slot Esirkepov algebra.
-//  * gather -> scatter: the adjoint of E_p, B_p is scattered onto the gather
-//    nodes of a ghosted FP64 array (atomics), then folded onto the interior;
-//    dE_p/dx comes from the derivative of the shape weights.
+//  * gather -> scatter: the adjoint of E_p, B_p is scattered (atomics) onto
+//    the interior partners of the gather nodes, the transpose of the ghost
+//    refresh the gather reads; dE_p/dx comes from the weight derivatives.
 //  * push: the 3x9 Jacobian of (v, E_p, B_p) -> v' by dual numbers, applied
 //    transposed.
 #include "pic_particle_math.h"
@@ -31,6 +32,17 @@ namespace {
 using namespace pic;
 
 constexpr int kThreads = 256;
+
+// periodic partner (interior index) of a ghosted index along one axis
+__device__ inline int partner(int r, int g, int n) {
+    int i = (r - g) % n;
+    return i < 0 ? i + n : i;
+}
+
+__device__ inline int64_t partner_index(const PicDesc& d, int i, int j, int k) {
+    return ((int64_t)partner(i, d.g, d.n[0]) * d.n[1] + partner(j, d.g, d.n[1])) * d.n[2] + partner(k, d.g, d.n[2]);
+}
+
 
 // ------------------------------------------------------------------ duals
 template <int N>
@@ -299,7 +311,7 @@ __device__ inline void scatter_adj(const PicDesc& d, const AdjGrid& gr, int stag
         for (int j = 0; j < ay.n; ++j)
             for (int k = 0; k < az.n; ++k) {
                 double w = ax.w[i] * ay.w[j] * az.w[k];
-                if (w != 0.0) unsafeAtomicAdd(&G[fidx(d, ax.p[i], ay.p[j], az.p[k])], lam * w);
+                if (w != 0.0) unsafeAtomicAdd(&G[partner_index(d, ax.p[i], ay.p[j], az.p[k])], lam * w);
             }
 }
 
@@ -314,6 +326,7 @@ using D6 = Dual<6>;
 
 struct DualAxis {
     int base;
+    int lo, hi;        // structural support of S0 and S_new in the 5-slot frame
     D6 S0[5], D[5], C[5];
 };
 
@@ -350,19 +363,33 @@ __device__ inline void dual_axis(const D6& xo, const D6& xn, double org, double 
         D6 tn = S1[0] + S1[1] + S1[2] + S1[3] + S1[4];
         D6 to = e.S0[0] + e.S0[1] + e.S0[2] + e.S0[3] + e.S0[4];
         e.base = g;
+        e.lo = e.hi = 0;
         e.S0[0] = to;
         e.D[0] = tn - to;
         e.C[0] = zero;
         return;
     }
     e.base = an - 2;
+    // S_new lives on slots {2,3} (CIC) / {1,2,3} (TSC); S0 on the same slots
+    // moved by -shift; a roll by two or more wraps, so keep the whole frame then
+    const int s_lo = sf == 1 ? 2 : 1;
+    if (shift > 1 || shift < -1) {
+        e.lo = 0;
+        e.hi = 4;
+    } else {
+        e.lo = min(s_lo, s_lo - shift);
+        e.hi = max(3, 3 - shift);
+    }
     for (int i = 0; i < 5; ++i) {
         run = run + e.D[i];
         e.C[i] = run;
     }
 }
 
-// Contracts lamJ (ghosted, the adjoint of the raw deposit) with dJ/d(x_old, v').
+// Contracts the adjoint of J (interior array; a raw deposit node reads its
+// periodic partner, the transpose of fold + refresh) with dJ/d(x_old, v').
+// Loops run over the structural support of the stencil: terms outside it are
+// constant zero in the forward deposit.
 __device__ inline void esirkepov_adjoint(const PicDesc& d, double qw, const D6 xo[3], const D6 xn[3], const D6 v[3],
                                          const double* const lamJ[3], double g6[6]) {
     DualAxis ax[3];
@@ -378,20 +405,23 @@ __device__ inline void esirkepov_adjoint(const PicDesc& d, double qw, const D6 x
         for (int a = 0; a < 3; ++a)
             if (a != c && act[a]) fa[nf++] = a;
         D6 pre;
-        int ni;
+        int i0, i1;
         if (act[c]) {
             pre = dconst<6>(-(qw / (d.d[o1] * d.d[o2])) / d.dt);
-            ni = 5;
+            i0 = ax[c].lo;
+            i1 = ax[c].hi;          // the running sum vanishes at the top of the support
         } else {
             pre = v[c] * (qw / dV);
-            ni = 1;
+            i0 = 0;
+            i1 = 1;
         }
-        const int nj = nf >= 1 ? 5 : 1, nk = nf == 2 ? 5 : 1;
+        const int j0 = nf >= 1 ? ax[fa[0]].lo : 0, j1 = nf >= 1 ? ax[fa[0]].hi : 0;
+        const int k0 = nf == 2 ? ax[fa[1]].lo : 0, k1 = nf == 2 ? ax[fa[1]].hi : 0;
         const double* L = lamJ[c];
-        for (int i = 0; i < ni; ++i) {
+        for (int i = i0; i < i1; ++i) {
             D6 ci = act[c] ? ax[c].C[i] : dconst<6>(1.0);
-            for (int j = 0; j < nj; ++j)
-                for (int k = 0; k < nk; ++k) {
+            for (int j = j0; j <= j1; ++j)
+                for (int k = k0; k <= k1; ++k) {
                     int idx[3] = {ax[0].base, ax[1].base, ax[2].base};
                     if (act[c]) idx[c] += i;
                     if (nf >= 1) idx[fa[0]] += j;
@@ -399,7 +429,7 @@ __device__ inline void esirkepov_adjoint(const PicDesc& d, double qw, const D6 x
                     if (!drop_index(idx[0], d.dims[0]) || !drop_index(idx[1], d.dims[1]) ||
                         !drop_index(idx[2], d.dims[2]))
                         continue;
-                    double lam = L[fidx(d, idx[0], idx[1], idx[2])];
+                    double lam = L[partner_index(d, idx[0], idx[1], idx[2])];
                     if (lam == 0.0) continue;
                     D6 y;
                     if (nf == 0) {
@@ -580,12 +610,6 @@ __global__ void scale_kernel(W3 out, D3 in, double coef, int64_t n) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     for (int c = 0; c < 3; ++c) out.p[c][t] = coef * in.p[c][t];
-}
-
-// periodic partner (interior index) of a ghosted index along one axis
-__device__ inline int partner(int r, int g, int n) {
-    int i = (r - g) % n;
-    return i < 0 ? i + n : i;
 }
 
 __global__ void expand_kernel(PicDesc d, const double* in, double* out, int64_t tot) {

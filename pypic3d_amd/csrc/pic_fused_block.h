@@ -207,6 +207,48 @@ __device__ inline void crossing_deposit(const PicDesc& d, double qw, const Cross
     crossing_component<2>(d, qw, cz, cx, cy, Jz);
 }
 
+// Face-crossing deposit with reduced axes (Esirkepov.py:241-329 in the
+// 3-slot frame): kDims = 2 (x, y active, z reduced) or 1 (x active).  The
+// active components sum the running differences C of their own axis times the
+// factor of the other active axes (S0 + D/2 in 2-D, 1 in 1-D); a reduced
+// component c carries q v_c / dV times the factor of all active axes.
+template <int kDims>
+__device__ inline void crossing_deposit_reduced(const PicDesc& d, const SpeciesK& k, const float v[3],
+                                                const CrossAxis& cx, const CrossAxis& cy, float* Jx, float* Jy,
+                                                float* Jz) {
+    const int g = d.g;
+    if constexpr (kDims == 2) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const float cix = i == 0 ? cx.C0 : cx.C1, ciy = i == 0 ? cy.C0 : cy.C1;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                if (cix != 0.0f) deposit_at(Jx, d, cx.b + i, cy.b + j, g, k.pre[0] * cix * (cy.S0[j] + 0.5f * cy.D[j]));
+                if (ciy != 0.0f) deposit_at(Jy, d, cx.b + j, cy.b + i, g, k.pre[1] * ciy * (cx.S0[j] + 0.5f * cx.D[j]));
+            }
+        }
+        const float pz = k.q_dv * v[2];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) {
+                float y = cx.S0[j] * cy.S0[kk] + 0.5f * (cx.S0[j] * cy.D[kk] + cx.D[j] * cy.S0[kk]) +
+                          (1.0f / 3.0f) * cx.D[j] * cy.D[kk];
+                deposit_at(Jz, d, cx.b + j, cy.b + kk, g, pz * y);
+            }
+    } else {
+        deposit_at(Jx, d, cx.b, g, g, k.pre[0] * cx.C0);
+        deposit_at(Jx, d, cx.b + 1, g, g, k.pre[0] * cx.C1);
+        const float py = k.q_dv * v[1], pz = k.q_dv * v[2];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const float f1 = cx.S0[j] + 0.5f * cx.D[j];
+            deposit_at(Jy, d, cx.b + j, g, g, py * f1);
+            deposit_at(Jz, d, cx.b + j, g, g, pz * f1);
+        }
+    }
+}
+
 // Per-block LDS: the face-crossing queue and the species constants.
 template <int kRounds>
 struct FusedShared {
@@ -534,6 +576,22 @@ __device__ inline void fused_block(const PicDesc& d, const PicSpecies& sp, const
             ok = cross_axis(xo[1], xn[1], d.org_c[1], d.sp_c[1], 1.0 / d.d[1], ac[1].org_c, ac[1].isp_c, cy) && ok;
             ok = cross_axis(xo[2], xn[2], d.org_c[2], d.sp_c[2], 1.0 / d.d[2], ac[2].org_c, ac[2].isp_c, cz) && ok;
             if (ok) crossing_deposit(d, k.qw, cx, cy, cz, Jx, Jy, Jz);
+            else esirkepov_particle(d, k.qw, xo, xn, v, J);   // moved two or more cells
+        }
+    } else if constexpr (kDims == 2 || kDims == 1) {
+        for (int q = threadIdx.x; q < nq; q += kThreads) {
+            uint32_t i = base + (uint32_t)sh.q_idx[q];
+            const SpeciesK& k = sh.spk[species_of(i, sp.slots, sp.count)];
+            float v[3] = {sh.q_v[0][q], sh.q_v[1][q], sh.q_v[2][q]};
+            double xo[3], xn[3];
+            load_pos(in, i, xo);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) xn[a] = ((k.ux >> a) & 1u) ? step_pos(xo[a], (double)v[a], d.dt) : xo[a];
+            CrossAxis cx, cy;
+            bool ok = cross_axis(xo[0], xn[0], d.org_c[0], d.sp_c[0], 1.0 / d.d[0], ac[0].org_c, ac[0].isp_c, cx);
+            if (kDims == 2)
+                ok = cross_axis(xo[1], xn[1], d.org_c[1], d.sp_c[1], 1.0 / d.d[1], ac[1].org_c, ac[1].isp_c, cy) && ok;
+            if (ok) crossing_deposit_reduced<kDims>(d, k, v, cx, cy, Jx, Jy, Jz);
             else esirkepov_particle(d, k.qw, xo, xn, v, J);   // moved two or more cells
         }
     } else {

@@ -122,12 +122,18 @@ def _gpu_grads(static, dyn, cfg, ppc, drift, amp, steps, k, seed=5):
     return float(loss), float(gd), float(ga), gv.cpu().numpy()
 
 
-@pytest.mark.parametrize("pusher,sf", [("boris", 1), ("higuera_cary", 1), ("boris", 2)])
-def test_field_energy_gradients_match_oracle_central_differences(pusher, sf):
+# The loss is piecewise smooth in the particle state: a first-order (CIC)
+# weight has a kink where a particle crosses a node, and a central difference
+# whose stencil straddles kinks carries an O(h) error from them (measured
+# ~2e-4 relative at h = 1e-4 on this case).  The second-order shape is C1, so
+# h = 1e-4 is used there; CIC is checked at h = 1e-6, where the FP64 oracle's
+# rounding contributes ~1e-9.
+@pytest.mark.parametrize("pusher,sf,h", [("boris", 1, 1e-6), ("higuera_cary", 1, 1e-6), ("boris", 2, 1e-4)])
+def test_field_energy_gradients_match_oracle_central_differences(pusher, sf, h):
     static, dyn, cfg = _setup(pusher=pusher, sf=sf)
     p = oracle_params(static, dyn)
     sp = oracle_species(cfg)
-    ppc, drift, amp, steps, h = 2, 0.1, 0.05, 12, 1e-4
+    ppc, drift, amp, steps = 2, 0.1, 0.05, 12
     loss, gd, ga, gv = _gpu_grads(static, dyn, cfg, ppc, drift, amp, steps, k=5)
 
     def L(dr, am, dv=None):
