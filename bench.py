@@ -227,6 +227,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--ppc", type=int, default=32)
     ap.add_argument("--sort-every", type=int, default=10)
+    ap.add_argument("--tile", type=int, default=8, choices=(0, 4, 8),
+                    help="cells per tile edge of the tile-ordered fused kernel (0: slot-ordered kernel)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--variant", type=int, default=None,
@@ -262,7 +264,8 @@ def main():
     static, dyn, species = workloads.config_warm_3d(n=args.n, ppc=args.ppc, vth=0.05)
     parts, cfg = workloads.plasma(static, dyn, species, seed=20261202)
     fields = workloads.empty_fields(static, dyn)
-    runner = ElectrodynamicRunner(parts, cfg, fields, static, dyn, sort_every=args.sort_every)
+    runner = ElectrodynamicRunner(parts, cfg, fields, static, dyn, sort_every=args.sort_every,
+                                  tile=args.tile or None)
     del parts
     torch.cuda.synchronize()
     n_particles = runner.S * runner.P
@@ -273,10 +276,14 @@ def main():
         runner.step()
     torch.cuda.synchronize()
     if args.variant is not None:
-        runner.desc.variant = variant = args.variant
+        if runner.tile:
+            runner.rounds = variant = args.variant
+        else:
+            runner.desc.variant = variant = args.variant
         tune_ms = {}
     else:
-        variant, tune_ms = runner.autotune() if not args.no_autotune else (0, {})
+        variant, tune_ms = runner.autotune() if not args.no_autotune else \
+            ((runner.rounds, runner.min_blocks) if runner.tile else 0, {})
     print(f"[bench] autotune fused variants (ms): {tune_ms} -> {variant}", file=sys.stderr, flush=True)
     runner.step()
     torch.cuda.synchronize()
@@ -317,10 +324,11 @@ def main():
                         "CIC Esirkepov, relativistic Boris, Yee",
             "particles": n_particles, "cells": n_cells, "sort_every": args.sort_every,
             "resorted_particles_per_step": n_particles * sorts / args.steps,
-            "fused_variant": variant, "autotune_ms": tune_ms,
+            "tile": runner.tile, "fused_variant": variant, "autotune_ms": tune_ms,
         },
         "roofline": {
-            "bound": "hbm", "kernel": "fused_push_esirkepov_kernel",
+            "bound": "hbm",
+            "kernel": "fused_tiled_kernel" if runner.tile else "fused_push_esirkepov_v2",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": None, "kernel_ms": kern_ms, "algorithmic_bytes": kern_bytes,

@@ -139,11 +139,35 @@ int pic_fused_push_esirkepov(const PicDesc* desc, const PicSpecies* sp, const Pi
                              const float* const E_ext[3], const float* const B_ext[3],
                              PicParticles* out, float* const J[3], void* stream);
 
+/* Tile-ordered variant for particles sorted by pic_tile_keys (3-D first-order
+ * case only): one block per (species, tile) slot range [starts[b], starts[b+1])
+ * from pic_tile_ranges accumulates its J contributions in an LDS box around the
+ * tile and adds the box into J once.  Same results as
+ * pic_fused_push_esirkepov up to FP32 summation order.  E+E_ext and B+B_ext
+ * around the tile are staged in LDS for the gather.  tile: 4 or 8 cells per
+ * axis; rounds: 2 or 4 particle slots per thread per pass; min_blocks: 1, 3
+ * or 4 resident blocks per CU requested from the register allocator. */
+int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                                   const float* const E[3], const float* const B[3], const float* const E_ext[3],
+                                   const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,
+                                   const int32_t* starts, int32_t rounds, int32_t min_blocks, void* stream);
+
 /* ---- cell sort (particle_initialization.py:89-102, 140-160) ------------- */
 /* key = s*(N+1) + (ix*Ny+iy)*Nz+iz with ix = clip(floor((x+W/2)/dx),0,Nx-1);
  * inactive slots get key s*(N+1) + N. */
 int pic_cell_keys(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
                   int32_t* keys, void* stream);
+/* Tile-major key for the tile-ordered fused kernel (a reordering for
+ * locality only; results do not depend on particle order): the cell
+ * (ix, iy, iz) of pic_cell_keys, tiles of tile^3 cells in x-major order,
+ * cells x-major inside a tile, inactive slots last in their species. */
+int64_t pic_tile_count(const PicDesc* desc, int32_t tile);
+int pic_tile_keys(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, int32_t tile, int32_t* keys,
+                  void* stream);
+/* starts[s*(tiles+1) + t] = first sorted slot of tile t of species s (t =
+ * tiles: the inactive tail); starts[count*(tiles+1)] = count*slots. */
+int pic_tile_ranges(const PicDesc* desc, const PicSpecies* sp, int32_t tile, const int32_t* keys_sorted,
+                    int32_t* starts, void* stream);
 int64_t pic_sort_workspace_bytes(int64_t n);
 /* Stable radix sort of (key, slot) pairs: perm = stable argsort(keys);
  * keys_sorted receives the sorted keys. */

@@ -73,7 +73,12 @@ SIGNATURES = {
     "pic_deposit_direct": (c_int, [DP, SP, PP, P3, c_vp]),
     "pic_deposit_rho": (c_int, [DP, SP, PP, c_vp, c_vp]),
     "pic_fused_push_esirkepov": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, PP, P3, c_vp]),
+    "pic_fused_push_esirkepov_tiled": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, PP, P3, c_i32, c_vp, c_i32, c_i32,
+                                               c_vp]),
     "pic_cell_keys": (c_int, [DP, SP, PP, c_vp, c_vp]),
+    "pic_tile_count": (c_i64, [DP, c_i32]),
+    "pic_tile_keys": (c_int, [DP, SP, PP, c_i32, c_vp, c_vp]),
+    "pic_tile_ranges": (c_int, [DP, SP, c_i32, c_vp, c_vp, c_vp]),
     "pic_sort_workspace_bytes": (c_i64, [c_i64]),
     "pic_sort_permutation": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i32, c_vp, c_i64, c_vp]),
     "pic_permute_particles": (c_int, [PP, PP, c_vp, c_i64, c_vp]),

@@ -73,15 +73,13 @@ __global__ __launch_bounds__(kThreads, 1) void batch_fused_kernel(const PicBatch
         const PicParticles in = offset_particles(args.in, poff);
         const PicParticles out = offset_particles(args.out, poff);
         const FieldPtrs fp = batch_field_ptrs(args.E, args.B, foff);
-        float* Jx = args.J[0] + foff;
-        float* Jy = args.J[1] + foff;
-        float* Jz = args.J[2] + foff;
+        const GlobalSink sink{{args.J[0] + foff, args.J[1] + foff, args.J[2] + foff}};
         const uint32_t base = blk * (uint32_t)(kThreads * kRounds);
         switch (fused_kind(d)) {
-            case 3: fused_block<3, kRounds>(d, bs.species, in, fp, out, Jx, Jy, Jz, n, base, sh); break;
-            case 2: fused_block<2, kRounds>(d, bs.species, in, fp, out, Jx, Jy, Jz, n, base, sh); break;
-            case 1: fused_block<1, kRounds>(d, bs.species, in, fp, out, Jx, Jy, Jz, n, base, sh); break;
-            default: fused_block<0, kRounds>(d, bs.species, in, fp, out, Jx, Jy, Jz, n, base, sh); break;
+            case 3: fused_block<3, kRounds>(d, bs.species, in, fp, out, sink, n, base, sh); break;
+            case 2: fused_block<2, kRounds>(d, bs.species, in, fp, out, sink, n, base, sh); break;
+            case 1: fused_block<1, kRounds>(d, bs.species, in, fp, out, sink, n, base, sh); break;
+            default: fused_block<0, kRounds>(d, bs.species, in, fp, out, sink, n, base, sh); break;
         }
         __syncthreads();   // the next item reuses the shared queue
     }

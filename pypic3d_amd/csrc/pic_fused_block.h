@@ -137,6 +137,33 @@ __device__ inline float gather_cic(const float* __restrict__ f, const float* __r
     return ax.w0 * (ay.w0 * c00 + ay.w1 * c01) + ax.w1 * (ay.w0 * c10 + ay.w1 * c11);
 }
 
+// Deposit targets.  GlobalSink adds straight into the J arrays (JAX scatter
+// "drop" index semantics, Esirkepov.py:232-240).  BoxSink accumulates into an
+// LDS box of B^3 nodes per component whose origin is `o` (array indices) and
+// falls back to the global arrays for nodes outside it; the caller flushes
+// the box into J with the same drop semantics.
+struct GlobalSink {
+    float* J[3];
+    __device__ inline void add(const PicDesc& d, int c, int i, int j, int k, float v) const {
+        deposit_at(J[c], d, i, j, k, v);
+    }
+};
+
+struct BoxSink {
+    float* J[3];
+    float* box;
+    int o[3];
+    int B;
+    __device__ inline void add(const PicDesc& d, int c, int i, int j, int k, float v) const {
+        if (v == 0.0f) return;
+        const int bi = i - o[0], bj = j - o[1], bk = k - o[2];
+        if ((unsigned)bi < (unsigned)B && (unsigned)bj < (unsigned)B && (unsigned)bk < (unsigned)B)
+            atomicAdd(&box[((c * B + bi) * B + bj) * B + bk], v);
+        else
+            deposit_at(J[c], d, i, j, k, v);
+    }
+};
+
 // First-order Esirkepov for a particle whose anchor moves by at most one cell
 // per axis, in a 3-slot frame starting at b = min(a_old, a_new): S0, D = S1 -
 // S0 (FP64) and the running sum C of D at slots 0 and 1 (slot 2's sum is 0).
@@ -175,9 +202,9 @@ __device__ inline float sel3(int k, float a0, float a1, float a2) { return k == 
 // All 54 terms (component c, differenced slot i in {0,1}, 3x3 slots (j, k) of
 // the other two axes) of one particle, fully unrolled so every per-axis value
 // stays in registers.
-template <int kC>
+template <int kC, class Sink>
 __device__ inline void crossing_component(const PicDesc& d, double qw, const CrossAxis& A, const CrossAxis& P,
-                                          const CrossAxis& Q, float* J) {
+                                          const CrossAxis& Q, const Sink& sink) {
     constexpr int o1 = kC == 0 ? 1 : 0, o2 = kC == 2 ? 1 : 2;
     const float pre = (float)(-(qw / (d.d[o1] * d.d[o2])) / d.dt);
 #pragma unroll
@@ -194,17 +221,18 @@ __device__ inline void crossing_component(const PicDesc& d, double qw, const Cro
                 int ix = kC == 0 ? ic : i1;
                 int iy = kC == 0 ? i1 : (kC == 1 ? ic : i2);
                 int iz = kC == 2 ? ic : i2;
-                deposit_at(J, d, ix, iy, iz, pre * ci * y);
+                sink.add(d, kC, ix, iy, iz, pre * ci * y);
             }
         }
     }
 }
 
+template <class Sink>
 __device__ inline void crossing_deposit(const PicDesc& d, double qw, const CrossAxis& cx, const CrossAxis& cy,
-                                        const CrossAxis& cz, float* Jx, float* Jy, float* Jz) {
-    crossing_component<0>(d, qw, cx, cy, cz, Jx);
-    crossing_component<1>(d, qw, cy, cx, cz, Jy);
-    crossing_component<2>(d, qw, cz, cx, cy, Jz);
+                                        const CrossAxis& cz, const Sink& sink) {
+    crossing_component<0>(d, qw, cx, cy, cz, sink);
+    crossing_component<1>(d, qw, cy, cx, cz, sink);
+    crossing_component<2>(d, qw, cz, cx, cy, sink);
 }
 
 // Face-crossing deposit with reduced axes (Esirkepov.py:241-329 in the
@@ -212,10 +240,9 @@ __device__ inline void crossing_deposit(const PicDesc& d, double qw, const Cross
 // active components sum the running differences C of their own axis times the
 // factor of the other active axes (S0 + D/2 in 2-D, 1 in 1-D); a reduced
 // component c carries q v_c / dV times the factor of all active axes.
-template <int kDims>
+template <int kDims, class Sink>
 __device__ inline void crossing_deposit_reduced(const PicDesc& d, const SpeciesK& k, const float v[3],
-                                                const CrossAxis& cx, const CrossAxis& cy, float* Jx, float* Jy,
-                                                float* Jz) {
+                                                const CrossAxis& cx, const CrossAxis& cy, const Sink& sink) {
     const int g = d.g;
     if constexpr (kDims == 2) {
 #pragma unroll
@@ -223,8 +250,8 @@ __device__ inline void crossing_deposit_reduced(const PicDesc& d, const SpeciesK
             const float cix = i == 0 ? cx.C0 : cx.C1, ciy = i == 0 ? cy.C0 : cy.C1;
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                if (cix != 0.0f) deposit_at(Jx, d, cx.b + i, cy.b + j, g, k.pre[0] * cix * (cy.S0[j] + 0.5f * cy.D[j]));
-                if (ciy != 0.0f) deposit_at(Jy, d, cx.b + j, cy.b + i, g, k.pre[1] * ciy * (cx.S0[j] + 0.5f * cx.D[j]));
+                if (cix != 0.0f) sink.add(d, 0, cx.b + i, cy.b + j, g, k.pre[0] * cix * (cy.S0[j] + 0.5f * cy.D[j]));
+                if (ciy != 0.0f) sink.add(d, 1, cx.b + j, cy.b + i, g, k.pre[1] * ciy * (cx.S0[j] + 0.5f * cx.D[j]));
             }
         }
         const float pz = k.q_dv * v[2];
@@ -234,19 +261,40 @@ __device__ inline void crossing_deposit_reduced(const PicDesc& d, const SpeciesK
             for (int kk = 0; kk < 3; ++kk) {
                 float y = cx.S0[j] * cy.S0[kk] + 0.5f * (cx.S0[j] * cy.D[kk] + cx.D[j] * cy.S0[kk]) +
                           (1.0f / 3.0f) * cx.D[j] * cy.D[kk];
-                deposit_at(Jz, d, cx.b + j, cy.b + kk, g, pz * y);
+                sink.add(d, 2, cx.b + j, cy.b + kk, g, pz * y);
             }
     } else {
-        deposit_at(Jx, d, cx.b, g, g, k.pre[0] * cx.C0);
-        deposit_at(Jx, d, cx.b + 1, g, g, k.pre[0] * cx.C1);
+        sink.add(d, 0, cx.b, g, g, k.pre[0] * cx.C0);
+        sink.add(d, 0, cx.b + 1, g, g, k.pre[0] * cx.C1);
         const float py = k.q_dv * v[1], pz = k.q_dv * v[2];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             const float f1 = cx.S0[j] + 0.5f * cx.D[j];
-            deposit_at(Jy, d, cx.b + j, g, g, py * f1);
-            deposit_at(Jz, d, cx.b + j, g, g, pz * f1);
+            sink.add(d, 1, cx.b + j, g, g, py * f1);
+            sink.add(d, 2, cx.b + j, g, g, pz * f1);
         }
     }
+}
+
+// LDS copy of E+E_ext and B+B_ext around a tile: component c occupies
+// box[c*B^3 ...], node (i, j, k) of the ghosted arrays at
+// ((i - o0)*B + (j - o1))*B + (k - o2), filled from index pmod(i, L) like the
+// global gather reads it.
+struct FieldBox {
+    const float* box;
+    int o[3];
+    int B;
+};
+
+__device__ inline float gather_box(const FieldBox& fb, int c, int a0, int a1, int a2, const CicAxis& ax,
+                                   const CicAxis& ay, const CicAxis& az) {
+    const int B = fb.B;
+    const float* p = fb.box + c * B * B * B + ((a0 - fb.o[0]) * B + (a1 - fb.o[1])) * B + (a2 - fb.o[2]);
+    const float v000 = p[0], v001 = p[1], v010 = p[B], v011 = p[B + 1];
+    const float v100 = p[B * B], v101 = p[B * B + 1], v110 = p[B * B + B], v111 = p[B * B + B + 1];
+    float c00 = v000 * az.w0 + v001 * az.w1, c01 = v010 * az.w0 + v011 * az.w1;
+    float c10 = v100 * az.w0 + v101 * az.w1, c11 = v110 * az.w0 + v111 * az.w1;
+    return ax.w0 * (ay.w0 * c00 + ay.w1 * c01) + ax.w1 * (ay.w0 * c10 + ay.w1 * c11);
 }
 
 // Per-block LDS: the face-crossing queue and the species constants.
@@ -269,21 +317,21 @@ struct RegularLayout {
         if (kDims == 2) return m < 2 ? 0 : (m < 4 ? 1 : 2);
         return m < 1 ? 0 : (m < 3 ? 1 : 2);
     }
-    __device__ static inline uint32_t off(int m, uint32_t sx, uint32_t sy) {
+    // node offset (dx, dy, dz) of value m from the particle's anchor cell
+    __device__ static inline void delta(int m, int& dx, int& dy, int& dz) {
+        dx = dy = dz = 0;
         if (kDims == 3) {
             const int q = m & 3, hi = q >> 1, lo = q & 1;
-            if (m < 4) return (hi ? sy : 0u) + (lo ? 1u : 0u);      // Jx: (a, b..b+1, c..c+1)
-            if (m < 8) return (hi ? sx : 0u) + (lo ? 1u : 0u);      // Jy: (a..a+1, b, c..c+1)
-            return (hi ? sx : 0u) + (lo ? sy : 0u);                 // Jz: (a..a+1, b..b+1, c)
+            if (m < 4) { dy = hi; dz = lo; }            // Jx: (a, b..b+1, c..c+1)
+            else if (m < 8) { dx = hi; dz = lo; }       // Jy: (a..a+1, b, c..c+1)
+            else { dx = hi; dy = lo; }                  // Jz: (a..a+1, b..b+1, c)
+        } else if (kDims == 2) {
+            if (m < 2) dy = m;                          // Jx: (a, b..b+1)
+            else if (m < 4) dx = m - 2;                 // Jy: (a..a+1, b)
+            else { dx = (m - 4) >> 1; dy = (m - 4) & 1; }   // Jz: (a..a+1, b..b+1)
+        } else if (m >= 1) {
+            dx = (m - 1) & 1;                           // Jy, Jz: (a..a+1)
         }
-        if (kDims == 2) {
-            if (m < 2) return m ? sy : 0u;                          // Jx: (a, b..b+1)
-            if (m < 4) return (m - 2) ? sx : 0u;                    // Jy: (a..a+1, b)
-            const int q = m - 4;
-            return ((q >> 1) ? sx : 0u) + ((q & 1) ? sy : 0u);      // Jz: (a..a+1, b..b+1)
-        }
-        if (m < 1) return 0u;                                       // Jx: (a)
-        return ((m - 1) & 1) ? sx : 0u;                             // Jy, Jz: (a..a+1)
     }
 };
 
@@ -328,10 +376,10 @@ __device__ inline void reduced_cic(int g, CicAxis& s) {
 // first-order fast path with that many leading active axes (x, y, z / x, y /
 // x); kDims = 0: any shape and reduced-axis layout through the general 5-slot
 // path.  `d` and `sp` may live in kernel arguments or in device memory.
-template <int kDims, int kRounds>
+template <int kDims, int kRounds, class Sink, bool kFieldBox = false>
 __device__ inline void fused_block(const PicDesc& d, const PicSpecies& sp, const PicParticles& in,
-                                   const FieldPtrs& fp, const PicParticles& out, float* Jx, float* Jy, float* Jz,
-                                   int64_t n, uint32_t base, FusedShared<kRounds>& sh) {
+                                   const FieldPtrs& fp, const PicParticles& out, const Sink& sink, int64_t n,
+                                   uint32_t base, FusedShared<kRounds>& sh, const FieldBox& fb = FieldBox{}) {
     constexpr bool kFast = kDims > 0;
     using Lay = RegularLayout<kDims == 0 ? 3 : kDims>;
     constexpr int NV = Lay::nv;
@@ -364,7 +412,7 @@ __device__ inline void fused_block(const PicDesc& d, const PicSpecies& sp, const
         float val[NV];
 #pragma unroll
         for (int m = 0; m < NV; ++m) val[m] = 0.0f;
-        int cell = 0;
+        int cell = 0, a0 = 0, a1 = 0, a2 = 0;
 
         if (act) {
             float xf[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
@@ -389,7 +437,22 @@ __device__ inline void fused_block(const PicDesc& d, const PicSpecies& sp, const
                         dc[a] = dv[a] = 0.0f;
                     }
                 }
-                if constexpr (kDims == 3) {
+                bool boxed = false;
+                if constexpr (kFieldBox && kDims == 3) {
+                    boxed = true;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+                        boxed = boxed && (unsigned)(anc[a] - fb.o[a]) < (unsigned)(fb.B - 1) &&
+                                (unsigned)(anv[a] - fb.o[a]) < (unsigned)(fb.B - 1);
+                }
+                if (boxed) {
+                    E[0] = gather_box(fb, 0, anv[0], anc[1], anc[2], gv[0], gc[1], gc[2]);
+                    E[1] = gather_box(fb, 1, anc[0], anv[1], anc[2], gc[0], gv[1], gc[2]);
+                    E[2] = gather_box(fb, 2, anc[0], anc[1], anv[2], gc[0], gc[1], gv[2]);
+                    B[0] = gather_box(fb, 3, anc[0], anv[1], anv[2], gc[0], gv[1], gv[2]);
+                    B[1] = gather_box(fb, 4, anv[0], anc[1], anv[2], gv[0], gc[1], gv[2]);
+                    B[2] = gather_box(fb, 5, anv[0], anv[1], anc[2], gv[0], gv[1], gc[2]);
+                } else if constexpr (kDims == 3) {
                     E[0] = gather_cic(fp.E[0], fp.Ee[0], d, gv[0], gc[1], gc[2]);
                     E[1] = gather_cic(fp.E[1], fp.Ee[1], d, gc[0], gv[1], gc[2]);
                     E[2] = gather_cic(fp.E[2], fp.Ee[2], d, gc[0], gc[1], gv[2]);
@@ -473,7 +536,9 @@ __device__ inline void fused_block(const PicDesc& d, const PicSpecies& sp, const
                             }
                         }
                     }
-                    cell = (int)fidx(d, anc[0], anc[1], anc[2]);
+                    a0 = anc[0];
+                    a1 = anc[1];
+                    a2 = anc[2];
                 } else if constexpr (kDims == 2) {
                     float px = k.pre[0] * (-D[0]), py = k.pre[1] * (-D[1]), pz = k.q_dv * v[2];
 #pragma unroll
@@ -488,7 +553,9 @@ __device__ inline void fused_block(const PicDesc& d, const PicSpecies& sp, const
                                                         (1.0f / 3.0f) * dxj * dyk);
                         }
                     }
-                    cell = (int)fidx(d, anc[0], anc[1], d.g);
+                    a0 = anc[0];
+                    a1 = anc[1];
+                    a2 = d.g;
                 } else {
                     float py = k.q_dv * v[1], pz = k.q_dv * v[2];
                     val[0] = k.pre[0] * (-D[0]);
@@ -498,8 +565,10 @@ __device__ inline void fused_block(const PicDesc& d, const PicSpecies& sp, const
                         val[1 + j] = py * f1;
                         val[3 + j] = pz * f1;
                     }
-                    cell = (int)fidx(d, anc[0], d.g, d.g);
+                    a0 = anc[0];
+                    a1 = a2 = d.g;
                 }
+                cell = (int)fidx(d, a0, a1, a2);
                 key = cell;
             } else {
                 int slot = atomicAdd(&sh.q_n, 1);
@@ -547,11 +616,12 @@ __device__ inline void fused_block(const PicDesc& d, const PicSpecies& sp, const
                 }
                 bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
                 if (tail && key >= 0) {
-                    const uint32_t c0 = (uint32_t)cell;
-                    float* const Jc[3] = {Jx, Jy, Jz};
 #pragma unroll
-                    for (int m = 0; m < NV; ++m)
-                        if (val[m] != 0.0f) atomic_add_f32(&Jc[Lay::comp(m)][c0 + Lay::off(m, sx, sy)], val[m]);
+                    for (int m = 0; m < NV; ++m) {
+                        int dx, dy, dz;
+                        Lay::delta(m, dx, dy, dz);
+                        sink.add(d, Lay::comp(m), a0 + dx, a1 + dy, a2 + dz, val[m]);
+                    }
                 }
             }
         }
@@ -560,7 +630,7 @@ __device__ inline void fused_block(const PicDesc& d, const PicSpecies& sp, const
 
     // ---- queued particles (Esirkepov.py:105-331)
     const int nq = sh.q_n;
-    float* J[3] = {Jx, Jy, Jz};
+    float* J[3] = {sink.J[0], sink.J[1], sink.J[2]};
     if constexpr (kDims == 3) {
         // one lane per queued particle, all terms unrolled (no scratch arrays)
         for (int q = threadIdx.x; q < nq; q += kThreads) {
@@ -575,7 +645,7 @@ __device__ inline void fused_block(const PicDesc& d, const PicSpecies& sp, const
             bool ok = cross_axis(xo[0], xn[0], d.org_c[0], d.sp_c[0], 1.0 / d.d[0], ac[0].org_c, ac[0].isp_c, cx);
             ok = cross_axis(xo[1], xn[1], d.org_c[1], d.sp_c[1], 1.0 / d.d[1], ac[1].org_c, ac[1].isp_c, cy) && ok;
             ok = cross_axis(xo[2], xn[2], d.org_c[2], d.sp_c[2], 1.0 / d.d[2], ac[2].org_c, ac[2].isp_c, cz) && ok;
-            if (ok) crossing_deposit(d, k.qw, cx, cy, cz, Jx, Jy, Jz);
+            if (ok) crossing_deposit(d, k.qw, cx, cy, cz, sink);
             else esirkepov_particle(d, k.qw, xo, xn, v, J);   // moved two or more cells
         }
     } else if constexpr (kDims == 2 || kDims == 1) {
@@ -591,7 +661,7 @@ __device__ inline void fused_block(const PicDesc& d, const PicSpecies& sp, const
             bool ok = cross_axis(xo[0], xn[0], d.org_c[0], d.sp_c[0], 1.0 / d.d[0], ac[0].org_c, ac[0].isp_c, cx);
             if (kDims == 2)
                 ok = cross_axis(xo[1], xn[1], d.org_c[1], d.sp_c[1], 1.0 / d.d[1], ac[1].org_c, ac[1].isp_c, cy) && ok;
-            if (ok) crossing_deposit_reduced<kDims>(d, k, v, cx, cy, Jx, Jy, Jz);
+            if (ok) crossing_deposit_reduced<kDims>(d, k, v, cx, cy, sink);
             else esirkepov_particle(d, k.qw, xo, xn, v, J);   // moved two or more cells
         }
     } else {

@@ -35,6 +35,58 @@ __global__ void cell_keys_kernel(PicDesc d, PicSpecies sp, PicParticles in, uint
     keys[i] = (uint32_t)(s * (ncell + 1) + key);
 }
 
+// Tile-major cell key: cubic tiles of T cells per axis, tiles in x-major
+// order, cells inside a tile in x-major order; inactive slots get the key
+// after the last tile.  Same cell formula as cell_keys_kernel.
+__host__ __device__ inline void tile_counts(const PicDesc& d, int T, int nt[3]) {
+    for (int a = 0; a < 3; ++a) nt[a] = (d.n[a] + T - 1) / T;
+}
+
+__global__ void tile_keys_kernel(PicDesc d, PicSpecies sp, PicParticles in, int T, uint32_t* keys, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = species_of(i, sp.slots, sp.count);
+    int nt[3];
+    tile_counts(d, T, nt);
+    const int64_t vol = (int64_t)T * T * T;
+    const int64_t span = (int64_t)nt[0] * nt[1] * nt[2] * vol;
+    int64_t key = span;
+    if (in.active[i]) {
+        int c[3];
+        for (int a = 0; a < 3; ++a) {
+            double t = floor(((double)in.x[a][i] + 0.5 * d.wind[a]) / d.d[a]);
+            t = fmin(fmax(t, 0.0), (double)(d.n[a] - 1));
+            c[a] = (int)t;
+        }
+        int64_t tile = ((int64_t)(c[0] / T) * nt[1] + c[1] / T) * nt[2] + c[2] / T;
+        int64_t local = ((int64_t)(c[0] % T) * T + c[1] % T) * T + c[2] % T;
+        key = tile * vol + local;
+    }
+    keys[i] = (uint32_t)(s * (span + 1) + key);
+}
+
+// starts[b] = first sorted position whose key >= the first key of tile b,
+// b = s * (tiles + 1) + t (t = tiles: the inactive tail); starts[last] = n.
+__global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, int64_t n, int64_t span, int64_t vol,
+                                   int32_t tiles, int32_t species, int32_t* starts) {
+    int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nb = (int64_t)species * (tiles + 1);
+    if (b > nb) return;
+    if (b == nb) {
+        starts[b] = (int32_t)n;
+        return;
+    }
+    const int64_t s = b / (tiles + 1), t = b % (tiles + 1);
+    const uint64_t target = (uint64_t)(s * (span + 1) + t * vol);
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if ((uint64_t)keys[mid] < target) lo = mid + 1;
+        else hi = mid;
+    }
+    starts[b] = (int32_t)lo;
+}
+
 __global__ void permute_kernel(PicParticles in, PicParticles out, const int32_t* __restrict__ perm, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -58,6 +110,42 @@ size_t sort_storage(int64_t n, int bits) {
 }  // namespace
 
 extern "C" {
+
+int64_t pic_tile_count(const PicDesc* desc, int32_t tile) {
+    if (!desc || tile < 1) return -1;
+    int nt[3];
+    tile_counts(*desc, tile, nt);
+    return (int64_t)nt[0] * nt[1] * nt[2];
+}
+
+int pic_tile_keys(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, int32_t tile, int32_t* keys,
+                  void* stream) {
+    PIC_REQUIRE(desc && sp && in && keys && in->active && in->x[0] && in->x[1] && in->x[2] && tile >= 1);
+    PIC_REQUIRE(sp->count >= 1 && sp->count <= PIC_MAX_SPECIES);
+    const int64_t span = pic_tile_count(desc, tile) * tile * tile * tile;
+    PIC_REQUIRE((int64_t)sp->count * (span + 1) < ((int64_t)1 << 31));
+    int64_t n = (int64_t)sp->count * sp->slots;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(tile_keys_kernel, dim3(grid_1d(n, kBlock)), dim3(kBlock), 0, as_stream(stream), *desc, *sp, *in,
+                       (int)tile, reinterpret_cast<uint32_t*>(keys), n);
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+int pic_tile_ranges(const PicDesc* desc, const PicSpecies* sp, int32_t tile, const int32_t* keys_sorted,
+                    int32_t* starts, void* stream) {
+    PIC_REQUIRE(desc && sp && keys_sorted && starts && tile >= 1);
+    const int64_t tiles = pic_tile_count(desc, tile);
+    const int64_t vol = (int64_t)tile * tile * tile;
+    const int64_t n = (int64_t)sp->count * sp->slots;
+    PIC_REQUIRE(n < ((int64_t)1 << 31) && (int64_t)sp->count * (tiles * vol + 1) < ((int64_t)1 << 31));
+    const int64_t nb = (int64_t)sp->count * (tiles + 1) + 1;
+    hipLaunchKernelGGL(tile_ranges_kernel, dim3(grid_1d(nb, kBlock)), dim3(kBlock), 0, as_stream(stream),
+                       reinterpret_cast<const uint32_t*>(keys_sorted), n, tiles * vol, vol, (int32_t)tiles,
+                       (int32_t)sp->count, starts);
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
 
 int pic_cell_keys(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, int32_t* keys, void* stream) {
     PIC_REQUIRE(desc && sp && in && keys && in->active && in->x[0] && in->x[1] && in->x[2]);

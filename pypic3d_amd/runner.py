@@ -26,7 +26,11 @@ from .utils import is_all_zero
 class ElectrodynamicRunner:
     PHASES = ("particles", "current_fold", "yee")
 
-    def __init__(self, particles, species_config, fields, static_parameters, dynamic_parameters, sort_every=0):
+    def __init__(self, particles, species_config, fields, static_parameters, dynamic_parameters, sort_every=0,
+                 tile=None, rounds=4, min_blocks=1):
+        """tile (4 or 8): use the tile-ordered fused kernel (3-D first-order
+        Esirkepov only) with particles kept in tile-major cell order; the
+        particles are sorted once here and then every ``sort_every`` steps."""
         E, B, J, rho, phi, external_fields, pml_state, overflow = fields
         if pml_state is not None:
             raise NotImplementedError("PML is not implemented on the GPU path")
@@ -65,14 +69,11 @@ class ElectrodynamicRunner:
         self.sort_every = int(sort_every)
         self.steps_done = 0
         self.events = None
-        if self.sort_every:
-            n = self.S * self.P
-            self.keys = torch.empty(n, dtype=torch.int32, device=dev)
-            self.keys_sorted = torch.empty(n, dtype=torch.int32, device=dev)
-            self.perm = torch.empty(n, dtype=torch.int32, device=dev)
-            self.sort_ws = torch.empty(max(1, int(N.lib().pic_sort_workspace_bytes(n))), dtype=torch.uint8,
-                                       device=dev)
+        self.keys = None
         self.sorts_done = 0
+        self.tile, self.rounds, self.min_blocks, self.starts = None, int(rounds), int(min_blocks), None
+        if tile is not None:
+            self.set_tile(tile)
 
     # -------------------------------------------------------------- helpers
     def _pp(self, k):
@@ -107,8 +108,13 @@ class ElectrodynamicRunner:
             for c in self.J:
                 N.call("pic_fill_f32", N.ptr(c), c.numel(), 0.0, s)
             ev = self._mark("particles")
-            N.call("pic_fused_push_esirkepov", d, self.sp, self._pp(src), N.p3(E), N.p3(self.B), self.Ee, self.Be,
-                   self._pp(dst), N.p3(self.J), s)
+            if self.tile:
+                N.call("pic_fused_push_esirkepov_tiled", d, self.sp, self._pp(src), N.p3(E), N.p3(self.B), self.Ee,
+                       self.Be, self._pp(dst), N.p3(self.J), self.tile, N.ptr(self.starts), self.rounds,
+                       self.min_blocks, s)
+            else:
+                N.call("pic_fused_push_esirkepov", d, self.sp, self._pp(src), N.p3(E), N.p3(self.B), self.Ee,
+                       self.Be, self._pp(dst), N.p3(self.J), s)
             if ev:
                 ev[1].record()
             ev = self._mark("current_fold")
@@ -143,29 +149,71 @@ class ElectrodynamicRunner:
             self.sort()
 
     def autotune(self, reps=3):
-        """Pick the fastest fused-kernel launch shape on the current state.
-        The timed launches write the inactive particle set and J, which the
-        next step overwrites."""
-        from .autotune import tune_fused
+        """Pick the fastest launch shape of the fused kernel on the current
+        state: the slot-ordered kernel's variants, or in tile mode the tiled
+        kernel's slots per thread.  The timed launches write the inactive
+        particle set and J, which the next step overwrites."""
+        from .autotune import tune_fused, tune_tiled
         if not self.esirkepov:
             return None, {}
         src, dst = self.cur, 1 - self.cur
         E = self.E[self.ecur]
+        if self.tile:
+            best, times = tune_tiled(self.desc, self.sp, self._pp(src), E, self.B, self.Ee, self.Be, self._pp(dst),
+                                     self.J, self.tile, self.starts, reps=reps)
+            self.rounds, self.min_blocks = best
+            return best, times
         best, times = tune_fused(self.desc, self.sp, self._pp(src), E, self.B, self.Ee, self.Be, self._pp(dst),
                                  self.J, reps=reps)
         self.desc.variant = best
         return best, times
 
+    def _sort_buffers(self):
+        if getattr(self, "keys", None) is None:
+            n = self.S * self.P
+            dev = self.device
+            self.keys = torch.empty(n, dtype=torch.int32, device=dev)
+            self.keys_sorted = torch.empty(n, dtype=torch.int32, device=dev)
+            self.perm = torch.empty(n, dtype=torch.int32, device=dev)
+            self.sort_ws = torch.empty(max(1, int(N.lib().pic_sort_workspace_bytes(n))), dtype=torch.uint8,
+                                       device=dev)
+
+    def set_tile(self, tile):
+        """Switch to the tile-ordered kernel with `tile`^3-cell tiles (None:
+        the slot-ordered kernel) and sort the particles accordingly."""
+        if tile is None:
+            self.tile, self.starts = None, None
+            return
+        d = self.desc
+        if not (self.esirkepov and d.shape_factor == 1 and all(d.n[a] > 1 for a in range(3))):
+            raise ValueError("the tile-ordered kernel needs the 3-D first-order Esirkepov path")
+        if tile not in (4, 8):
+            raise ValueError("tile must be 4 or 8")
+        self.tile = int(tile)
+        tiles = int(N.lib().pic_tile_count(d, self.tile))
+        self.starts = torch.empty(self.S * (tiles + 1) + 1, dtype=torch.int32, device=self.device)
+        self.sort()
+
     def sort(self):
-        """Stable cell sort of the current particle buffers (into the other set)."""
+        """Stable cell sort of the current particle buffers (into the other
+        set): reference cell order, or tile-major order in tile mode."""
+        self._sort_buffers()
         s = N.stream_handle()
         src, dst = self.cur, 1 - self.cur
         n = self.S * self.P
-        ncell = self.desc.n[0] * self.desc.n[1] * self.desc.n[2]
-        N.call("pic_cell_keys", self.desc, self.sp, self._pp(src), N.ptr(self.keys), s)
-        N.call("pic_sort_permutation", N.ptr(self.keys), N.ptr(self.keys_sorted), N.ptr(self.perm), n,
-               key_bits(self.S, ncell), N.ptr(self.sort_ws), self.sort_ws.numel(), s)
+        if self.tile:
+            span = int(N.lib().pic_tile_count(self.desc, self.tile)) * self.tile ** 3
+            N.call("pic_tile_keys", self.desc, self.sp, self._pp(src), self.tile, N.ptr(self.keys), s)
+            bits = key_bits(self.S, span)
+        else:
+            ncell = self.desc.n[0] * self.desc.n[1] * self.desc.n[2]
+            N.call("pic_cell_keys", self.desc, self.sp, self._pp(src), N.ptr(self.keys), s)
+            bits = key_bits(self.S, ncell)
+        N.call("pic_sort_permutation", N.ptr(self.keys), N.ptr(self.keys_sorted), N.ptr(self.perm), n, bits,
+               N.ptr(self.sort_ws), self.sort_ws.numel(), s)
         N.call("pic_permute_particles", self._pp(src), self._pp(dst), N.ptr(self.perm), n, s)
+        if self.tile:
+            N.call("pic_tile_ranges", self.desc, self.sp, self.tile, N.ptr(self.keys_sorted), N.ptr(self.starts), s)
         self.cur = dst
         self.sorts_done += 1
 

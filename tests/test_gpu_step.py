@@ -180,28 +180,35 @@ def test_fused_step_on_cell_sorted_dense_plasma_matches_oracle(N, ppc, pusher):
         assert rel_err(host(fields[1][c]), want.B[c]) <= TOL, ("B", c)
 
 
-def test_runner_matches_time_loop_and_sorting_is_transparent():
+@pytest.mark.parametrize("tile,rounds,min_blocks", [(None, 2, 1), (8, 2, 3), (4, 2, 1), (8, 4, 1), (4, 4, 4)])
+def test_runner_matches_time_loop_and_sorting_is_transparent(tile, rounds, min_blocks):
+    """Slot-ordered and tile-ordered runners (re-sorting every 2 steps, the
+    tile grid not dividing the domain) against the functional step."""
     from pypic3d_amd.runner import ElectrodynamicRunner
     static, dyn = workloads.make_parameters((16, 12, 10), cfl=0.5)
-    parts, cfg = workloads.plasma(static, dyn, [workloads.SpeciesSpec(16, weight=0.01, vth=(0.1,) * 3)], seed=5)
+    specs = [workloads.SpeciesSpec(16, weight=0.01, vth=(0.1,) * 3),
+             workloads.SpeciesSpec(5, charge=1.0, mass=20.0, weight=0.03, vth=(0.02,) * 3)]
+    parts, cfg = workloads.plasma(static, dyn, specs, seed=5)
     fields = workloads.empty_fields(static, dyn)
     p = oracle_params(static, dyn)
     E0 = tuple(gpu_field(random_field(p, 60 + c, scale=0.1)) for c in range(3))
     fields = (E0,) + fields[1:]
-    runner = ElectrodynamicRunner(parts, cfg, fields, static, dyn, sort_every=2)
+    runner = ElectrodynamicRunner(parts, cfg, fields, static, dyn, sort_every=2, tile=tile, rounds=rounds,
+                                  min_blocks=min_blocks)
     ref_parts, ref_fields = parts, fields
-    for _ in range(3):
+    for _ in range(5):
         runner.step()
         ref_parts, ref_fields = time_loop_electrodynamic(ref_parts, cfg, ref_fields, static, dyn)
     r_parts, r_fields = runner.state()
     # the runner re-sorted its slots; compare particle sets ordered by position
     xa, ua, aa = host_particles(r_parts)
     xb, ub, ab = host_particles(ref_parts)
-    oa = np.lexsort((xa[0, :, 2], xa[0, :, 1], xa[0, :, 0]))
-    ob = np.lexsort((xb[0, :, 2], xb[0, :, 1], xb[0, :, 0]))
-    np.testing.assert_array_equal(aa[0, oa], ab[0, ob])
-    assert np.max(np.abs(xa[0, oa] - xb[0, ob])) <= 1e-5
-    assert rel_err(ua[0, oa], ub[0, ob]) <= 1e-5
+    for s_ in range(2):
+        oa = np.lexsort((xa[s_, :, 2], xa[s_, :, 1], xa[s_, :, 0]))
+        ob = np.lexsort((xb[s_, :, 2], xb[s_, :, 1], xb[s_, :, 0]))
+        np.testing.assert_array_equal(aa[s_, oa], ab[s_, ob])
+        assert np.max(np.abs(xa[s_, oa] - xb[s_, ob])) <= 1e-5
+        assert rel_err(ua[s_, oa], ub[s_, ob]) <= 1e-5
     for k in (0, 1, 2):
         for c in range(3):
             assert rel_err(host(r_fields[k][c]), host(ref_fields[k][c])) <= 1e-5, (k, c)
