@@ -253,11 +253,15 @@ int pic_adjoint_yee(const PicDesc* desc, double* const lamE[3], double* const la
  * lamJ: interior adjoint of J (read at each deposit node's periodic partner,
  * the transpose of fold + refresh); lamE, lamB: interior FP64 arrays to which
  * the adjoint of the gathered fields is added (atomics at the gather nodes'
- * periodic partners, the transpose of the ghost refresh). */
+ * periodic partners, the transpose of the ghost refresh).  min_blocks (1, 4 or
+ * 8): occupancy hint for the register allocator.  workspace: 1 + count*slots
+ * int32 (first-order particles that move two or more cells are finished by a
+ * second, general pass). */
 int pic_adjoint_particles(const PicDesc* desc, const PicSpecies* sp, const PicParticles* state,
                           const float* const E[3], const float* const B[3], const float* const E_ext[3],
                           const float* const B_ext[3], const double* const lamJ[3], double* const lam_x[3],
-                          double* const lam_v[3], double* const lamE[3], double* const lamB[3], void* stream);
+                          double* const lam_v[3], double* const lamE[3], double* const lamB[3], int32_t min_blocks,
+                          int32_t* workspace, void* stream);
 
 #ifdef __cplusplus
 }

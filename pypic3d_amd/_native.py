@@ -95,7 +95,7 @@ SIGNATURES = {
     "pic_periodic_expand_f64": (c_int, [DP, c_vp, c_vp, c_vp]),
     "pic_periodic_fold_add_f64": (c_int, [DP, c_vp, c_vp, c_vp]),
     "pic_adjoint_yee": (c_int, [DP, P3, P3, P3, c_vp * 6, c_vp]),
-    "pic_adjoint_particles": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, P3, P3, P3, P3, P3, c_vp]),
+    "pic_adjoint_particles": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, P3, P3, P3, P3, P3, c_i32, c_vp, c_vp]),
 }
 
 _lib = None

@@ -46,13 +46,14 @@ class AdjointRun:
     """Forward with checkpoints, then reverse sweep with recomputation."""
 
     def __init__(self, static_parameters, dynamic_parameters, species_config, active, external_fields=None,
-                 checkpoint_every=20):
+                 checkpoint_every=20, min_blocks=4):
         _check_supported(static_parameters, dynamic_parameters)
         self.static, self.dynamic = static_parameters, dynamic_parameters
         self.cfg = species_config
         self.active = active
         self.external = external_fields
         self.k = max(1, int(checkpoint_every))
+        self.min_blocks = int(min_blocks)
         self.runner = None
         self.steps = 0
 
@@ -128,6 +129,7 @@ class AdjointRun:
         work = torch.empty((6,) + tuple(lamE.shape[1:]), dtype=torch.float64, device=dev)
         lx = torch.zeros((3, r.S, r.P), dtype=torch.float64, device=dev)
         lv = torch.zeros_like(lx)
+        defer = torch.empty(1 + r.S * r.P, dtype=torch.int32, device=dev)
         p3 = lambda t: N.P3(*[t[c].data_ptr() for c in range(3)])   # noqa: E731
         w6 = (N.c_vp * 6)(*[work[c].data_ptr() for c in range(6)])
         LE, LB, LJ, LX, LV = p3(lamE), p3(lamB), p3(lamJ), p3(lx), p3(lv)
@@ -156,7 +158,7 @@ class AdjointRun:
                 pp = pic_particles([seg_x[m, 0], seg_x[m, 1], seg_x[m, 2]], [seg_v[m, 0], seg_v[m, 1], seg_v[m, 2]],
                                    seg_a[m])
                 N.call("pic_adjoint_particles", d, r.sp, pp, p3(seg_E[m]), p3(seg_B[m]), r.Ee, r.Be, LJ, LX, LV, LE,
-                       LB, s)
+                       LB, self.min_blocks, N.ptr(defer), s)
         return lx, lv, lamE, lamB
 
 

@@ -35,7 +35,9 @@ constexpr int kThreads = 256;
 
 // periodic partner (interior index) of a ghosted index along one axis
 __device__ inline int partner(int r, int g, int n) {
-    int i = (r - g) % n;
+    int i = r - g;
+    if ((unsigned)i < (unsigned)n) return i;   // interior: no wrap
+    i %= n;
     return i < 0 ? i + n : i;
 }
 
@@ -166,6 +168,8 @@ __device__ inline Dual<N> dsqrt(const Dual<N>& a) {
     return r;
 }
 
+__device__ inline double dsqrt(double a) { return sqrt(a); }
+
 template <class T>
 __device__ inline void tcross(const T a[3], const T b[3], T c[3]) {
     c[0] = a[1] * b[2] - a[2] * b[1];
@@ -242,6 +246,7 @@ __device__ inline void push_t(const T v[3], const T E[3], const T B[3], double h
 // points, weights and their x-derivatives (weights divide by the grid spacing).
 struct AdjAxis {
     int p[3];
+    int r0;                 // unwrapped array index of point 0 (the scatter box coordinate)
     double w[3], dw[3];
     int n;
 };
@@ -268,9 +273,11 @@ __device__ inline void adj_gather_axis(double x, double org, double sp, int sf, 
             s.w[q] = w[q];
             s.dw[q] = dw[q];
         }
+        s.r0 = a - 1;
         s.n = 3;
     } else {
         s.p[0] = g;
+        s.r0 = g;
         s.w[0] = w[0] + w[1] + w[2];
         s.dw[0] = 0.0;      // the collapsed weight is the partition of unity
         s.n = 1;
@@ -289,9 +296,13 @@ __device__ inline void gather_grad(const PicDesc& d, const AdjGrid& gr, int stag
     const AdjAxis& az = (stag & 4) ? gr.v[2] : gr.c[2];
     val = 0.0;
     grad[0] = grad[1] = grad[2] = 0.0;
-    for (int i = 0; i < ax.n; ++i)
-        for (int j = 0; j < ay.n; ++j)
-            for (int k = 0; k < az.n; ++k) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (i >= ax.n || j >= ay.n || k >= az.n) continue;
                 int64_t id = fidx(d, ax.p[i], ay.p[j], az.p[k]);
                 double f = (double)F[id];
                 if (Fe) f += (double)Fe[id];
@@ -302,16 +313,36 @@ __device__ inline void gather_grad(const PicDesc& d, const AdjGrid& gr, int stag
             }
 }
 
-__device__ inline void scatter_adj(const PicDesc& d, const AdjGrid& gr, int stag, double lam, double* G) {
+// Scatter box: the block's gather nodes (unwrapped indices) in LDS for the
+// six components; nodes are added to the interior adjoint arrays at their
+// periodic partners once per block.
+struct ScatterBox {
+    double* box;
+    int o[3], e[3];
+    bool on;
+};
+
+__device__ inline void scatter_adj(const PicDesc& d, const AdjGrid& gr, int stag, double lam, double* G,
+                                   const ScatterBox& sb, int comp) {
     if (lam == 0.0) return;
     const AdjAxis& ax = (stag & 1) ? gr.v[0] : gr.c[0];
     const AdjAxis& ay = (stag & 2) ? gr.v[1] : gr.c[1];
     const AdjAxis& az = (stag & 4) ? gr.v[2] : gr.c[2];
-    for (int i = 0; i < ax.n; ++i)
-        for (int j = 0; j < ay.n; ++j)
-            for (int k = 0; k < az.n; ++k) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (i >= ax.n || j >= ay.n || k >= az.n) continue;
                 double w = ax.w[i] * ay.w[j] * az.w[k];
-                if (w != 0.0) unsafeAtomicAdd(&G[partner_index(d, ax.p[i], ay.p[j], az.p[k])], lam * w);
+                if (w == 0.0) continue;
+                if (sb.on) {
+                    const int bi = ax.r0 + i - sb.o[0], bj = ay.r0 + j - sb.o[1], bk = az.r0 + k - sb.o[2];
+                    atomicAdd(&sb.box[((comp * sb.e[0] + bi) * sb.e[1] + bj) * sb.e[2] + bk], lam * w);
+                } else {
+                    unsafeAtomicAdd(&G[partner_index(d, ax.p[i], ay.p[j], az.p[k])], lam * w);
+                }
             }
 }
 
@@ -390,8 +421,8 @@ __device__ inline void dual_axis(const D6& xo, const D6& xn, double org, double 
 // periodic partner, the transpose of fold + refresh) with dJ/d(x_old, v').
 // Loops run over the structural support of the stencil: terms outside it are
 // constant zero in the forward deposit.
-__device__ inline void esirkepov_adjoint(const PicDesc& d, double qw, const D6 xo[3], const D6 xn[3], const D6 v[3],
-                                         const double* const lamJ[3], double g6[6]) {
+__device__ inline void esirkepov_adjoint(const PicDesc& d, double qw, const D6 xo[3], const D6 xn[3],
+                                               const D6 v[3], const double* const lamJ[3], double g6[6]) {
     DualAxis ax[3];
     bool act[3];
     for (int a = 0; a < 3; ++a) {
@@ -450,6 +481,126 @@ __device__ inline void esirkepov_adjoint(const PicDesc& d, double qw, const D6 x
     }
 }
 
+// First-order Esirkepov adjoint in the 3-slot frame of a particle whose
+// anchor moves by at most one cell per active axis (slots b = min(a_old,
+// a_new) .. b+2; all other slots of the 5-slot frame hold zero).  Weights are
+// linear in the positions, so the derivatives are exact: per axis, S0 depends
+// on x_old only and S_new on x_new only (dS/dx = +-1/d).  Returns false when
+// an axis moves further (the caller uses the general dual-number path).
+struct Cic3 {
+    int b;
+    float S0[3], D[3], dS0[3], dS1[3];    // dS0 = dS0/dx_old, dS1 = dS_new/dx_new
+};
+
+__device__ inline bool cic3_axis(double xo, double xn, double org, double sp, double dd, Cic3& e) {
+    const int ao = anchor_of(xo, org, sp, 1), an = anchor_of(xn, org, sp, 1);
+    if (an - ao > 1 || ao - an > 1) return false;
+    const double ro = (xo - node_coord(ao, sp, org)) / dd, rn = (xn - node_coord(an, sp, org)) / dd;
+    const int b = min(ao, an);
+    double s0[3] = {0.0, 0.0, 0.0}, s1[3] = {0.0, 0.0, 0.0};
+    const float id = (float)(1.0 / dd);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) e.dS0[q] = e.dS1[q] = 0.0f;
+    if (ao == b) { s0[0] = 1.0 - ro; s0[1] = ro; e.dS0[0] = -id; e.dS0[1] = id; }
+    else { s0[1] = 1.0 - ro; s0[2] = ro; e.dS0[1] = -id; e.dS0[2] = id; }
+    if (an == b) { s1[0] = 1.0 - rn; s1[1] = rn; e.dS1[0] = -id; e.dS1[1] = id; }
+    else { s1[1] = 1.0 - rn; s1[2] = rn; e.dS1[1] = -id; e.dS1[2] = id; }
+    e.b = b;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        e.S0[q] = (float)s0[q];
+        e.D[q] = (float)(s1[q] - s0[q]);
+    }
+    return true;
+}
+
+// Adds dL/dx_old (go[3]), dL/dx_new (gn[3]) and the direct dL/dv' of reduced
+// components (gv[3]) for lambda = the interior adjoint of J.
+__device__ inline bool cic_esirkepov_adjoint(const PicDesc& d, double qw, const double xo[3], const double xn[3],
+                                             const double v[3], const double* const lamJ[3], double go[3],
+                                             double gn[3], double gv[3]) {
+    Cic3 ax[3];
+    bool act[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        act[a] = d.n[a] > 1;
+        if (act[a] && !cic3_axis(xo[a], xn[a], d.org_c[a], d.sp_c[a], d.d[a], ax[a])) return false;
+        if (!act[a]) ax[a].b = d.g;
+    }
+    const double dV = d.d[0] * d.d[1] * d.d[2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int o1 = c == 0 ? 1 : 0, o2 = c == 2 ? 1 : 2;
+        // factor axes: the active axes among o1, o2 (in that order)
+        const bool f1 = act[o1], f2 = act[o2];
+        const int A = f1 ? o1 : o2, Bx = o2;
+        const int nf = (f1 ? 1 : 0) + (f2 ? 1 : 0);
+        const bool ac = act[c];
+        const double pre = ac ? -(qw / (d.d[o1] * d.d[o2])) / d.dt : qw * v[c] / dV;
+        const double* L = lamJ[c];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            if (!ac && i > 0) continue;
+            // running sum of D along c (slots 0, 1) and its derivatives
+            float C = 1.0f, dCo = 0.0f, dCn = 0.0f;
+            if (ac) {
+                C = ax[c].D[0] + (i ? ax[c].D[1] : 0.0f);
+                dCo = -(ax[c].dS0[0] + (i ? ax[c].dS0[1] : 0.0f));
+                dCn = ax[c].dS1[0] + (i ? ax[c].dS1[1] : 0.0f);
+            }
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                if (nf == 0 && j > 0) continue;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    if (nf < 2 && k > 0) continue;
+                    int idx[3] = {ax[0].b, ax[1].b, ax[2].b};
+                    if (ac) idx[c] += i;
+                    if (nf >= 1) idx[A] += j;
+                    if (nf == 2) idx[Bx] += k;
+                    if (!drop_index(idx[0], d.dims[0]) || !drop_index(idx[1], d.dims[1]) ||
+                        !drop_index(idx[2], d.dims[2]))
+                        continue;
+                    const double lam = L[partner_index(d, idx[0], idx[1], idx[2])];
+                    if (lam == 0.0) continue;
+                    float F = 1.0f, dFAo = 0.0f, dFAn = 0.0f, dFBo = 0.0f, dFBn = 0.0f;
+                    if (nf == 1) {
+                        const Cic3& P = ax[A];
+                        F = P.S0[j] + 0.5f * P.D[j];
+                        dFAo = 0.5f * P.dS0[j];
+                        dFAn = 0.5f * P.dS1[j];
+                    } else if (nf == 2) {
+                        const Cic3& P = ax[A];
+                        const Cic3& Q = ax[Bx];
+                        F = P.S0[j] * Q.S0[k] + 0.5f * (P.S0[j] * Q.D[k] + P.D[j] * Q.S0[k]) +
+                            (1.0f / 3.0f) * P.D[j] * Q.D[k];
+                        dFAo = P.dS0[j] * (0.5f * Q.S0[k] + Q.D[k] * (1.0f / 6.0f));
+                        dFAn = P.dS1[j] * (0.5f * Q.S0[k] + Q.D[k] * (1.0f / 3.0f));
+                        dFBo = Q.dS0[k] * (0.5f * P.S0[j] + P.D[j] * (1.0f / 6.0f));
+                        dFBn = Q.dS1[k] * (0.5f * P.S0[j] + P.D[j] * (1.0f / 3.0f));
+                    }
+                    const double lp = lam * pre;
+                    if (ac) {
+                        go[c] += lp * dCo * F;
+                        gn[c] += lp * dCn * F;
+                    } else {
+                        gv[c] += lam * (qw / dV) * F;
+                    }
+                    if (nf >= 1) {
+                        go[A] += lp * C * dFAo;
+                        gn[A] += lp * C * dFAn;
+                    }
+                    if (nf == 2) {
+                        go[Bx] += lp * C * dFBo;
+                        gn[Bx] += lp * C * dFBn;
+                    }
+                }
+            }
+        }
+    }
+    return true;
+}
+
 // ----------------------------------------------------------- particle kernel
 struct LamPtrs {
     const double* J[3];
@@ -459,90 +610,193 @@ struct LamPtrs {
     double* B[3];
 };
 
-__global__ void adjoint_particles_kernel(PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, LamPtrs lp,
-                                         int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (!in.active[i]) {
-        for (int a = 0; a < 3; ++a) {
-            lp.x[a][i] = 0.0;
-            lp.v[a][i] = 0.0;
-        }
-        return;
-    }
-    const int s = species_of(i, sp.slots, sp.count);
-    const SpeciesK k = species_constants(d, sp, s);
-    double x[3], v[3], lx[3], lv[3];
-    for (int a = 0; a < 3; ++a) {
-        x[a] = (double)in.x[a][i];
-        v[a] = (double)in.u[a][i];
-        lx[a] = lp.x[a][i];          // adjoint of x' (the wrap is the identity map's derivative)
-        lv[a] = lp.v[a][i];          // adjoint of v'
-    }
-    // gather with gradients
+constexpr int kAdjThreads = 128;
+
+// Gather values (no derivatives) of the six components at x.
+__device__ inline void gather_values(const PicDesc& d, const FieldPtrs& fp, const double x[3], double Ep[3],
+                                     double Bp[3]) {
     AdjGrid gr;
     for (int a = 0; a < 3; ++a) {
         bool act = d.n[a] > 1;
         adj_gather_axis(x[a], d.org_c[a], d.sp_c[a], d.shape_factor, d.dims[a], act, d.g, gr.c[a]);
         adj_gather_axis(x[a], d.org_v[a], d.sp_v[a], d.shape_factor, d.dims[a], act, d.g, gr.v[a]);
     }
-    double Ep[3], Bp[3], dE[3][3], dB[3][3];
+    double g3[3];
     for (int c = 0; c < 3; ++c) {
-        gather_grad(d, gr, kStagE[c], fp.E[c], fp.Ee[c], Ep[c], dE[c]);
-        gather_grad(d, gr, kStagB[c], fp.B[c], fp.Be[c], Bp[c], dB[c]);
-    }
-    // push and its Jacobian over (v, E_p, B_p)
-    Dual<9> vv[3], EE[3], BB[3], vn[3];
-    for (int a = 0; a < 3; ++a) {
-        vv[a] = dvar<9>(v[a], a);
-        EE[a] = dvar<9>(Ep[a], 3 + a);
-        BB[a] = dvar<9>(Bp[a], 6 + a);
-    }
-    const double h = sp.charge[s] * d.dt / (2.0 * sp.mass[s]);
-    push_t(vv, EE, BB, h, d.pusher, d.relativistic, d.C, vn);
-    double vp[3];
-    for (int a = 0; a < 3; ++a) vp[a] = ((k.uu >> a) & 1u) ? vn[a].v : v[a];
-    // position update x' = x + v' dt on update_x axes
-    for (int a = 0; a < 3; ++a)
-        if ((k.ux >> a) & 1u) lv[a] += d.dt * lx[a];
-    // deposit -> gather of the adjoint of J
-    D6 xo[3], xnew[3], vd[3];
-    for (int a = 0; a < 3; ++a) {
-        xo[a] = dvar<6>(x[a], a);
-        vd[a] = dvar<6>(vp[a], 3 + a);
-        if ((k.ux >> a) & 1u) {
-            xnew[a] = xo[a] + vd[a] * d.dt;
-            xnew[a].v = step_pos(x[a], vp[a], d.dt);    // same rounding as the forward anchor
-        } else {
-            xnew[a] = xo[a];
-        }
-    }
-    double g6[6] = {0, 0, 0, 0, 0, 0};
-    esirkepov_adjoint(d, k.qw, xo, xnew, vd, lp.J, g6);
-    for (int a = 0; a < 3; ++a) {
-        lx[a] += g6[a];
-        lv[a] += g6[3 + a];
-    }
-    // push transposed: lambda(v, E_p, B_p) = J^T lambda(v') on update_u axes
-    double lin[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int a = 0; a < 3; ++a) {
-        if ((k.uu >> a) & 1u) {
-            for (int m = 0; m < 9; ++m) lin[m] += lv[a] * vn[a].t[m];
-        } else {
-            lin[a] += lv[a];
-        }
-    }
-    // gather -> scatter, and the weight derivatives
-    for (int c = 0; c < 3; ++c) {
-        scatter_adj(d, gr, kStagE[c], lin[3 + c], lp.E[c]);
-        scatter_adj(d, gr, kStagB[c], lin[6 + c], lp.B[c]);
-        for (int a = 0; a < 3; ++a) lx[a] += lin[3 + c] * dE[c][a] + lin[6 + c] * dB[c][a];
-    }
-    for (int a = 0; a < 3; ++a) {
-        lp.x[a][i] = lx[a];
-        lp.v[a][i] = lin[a];
+        gather_grad(d, gr, kStagE[c], fp.E[c], fp.Ee[c], Ep[c], g3);
+        gather_grad(d, gr, kStagB[c], fp.B[c], fp.Be[c], Bp[c], g3);
     }
 }
+
+// The phases keep few values live at a time (register pressure sets the
+// occupancy of this FP64 kernel): (1) gather values, push values, position
+// and Esirkepov adjoints -> lambda(x), lambda(v'); (2) push Jacobian, one
+// tangent direction per pass, contracted with lambda(v') -> lambda(v, E_p,
+// B_p); (3) gather stencils again for dE_p/dx and the scatter of lambda(E_p,
+// B_p) through the block's LDS box.
+// kAnalytic: the first-order 3-slot path; a particle that moves two or more
+// cells on an axis is appended to `defer` (untouched here) for the general
+// pass.  !kAnalytic: the general dual-number path, over all slots (only ==
+// nullptr) or over the `*count` slots listed in `only`.
+template <bool kAnalytic, int kMinBlocks>
+__global__ __launch_bounds__(kAdjThreads, kMinBlocks) void adjoint_particles_kernel(
+    PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, LamPtrs lp, int64_t n, int32_t* defer,
+    int32_t* defer_count, const int32_t* only, const int32_t* only_count) {
+    constexpr int kCap = 3072;   // doubles of LDS for the scatter box (6 components)
+    __shared__ double sbox[kCap];
+    __shared__ int bmin[3], bmax[3];
+    const int64_t m_all = only ? (int64_t)*only_count : n;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < m_all; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = base + threadIdx.x;
+    const int64_t i = t < m_all ? (only ? (int64_t)only[t] : t) : n;
+    if (threadIdx.x < 3) {
+        bmin[threadIdx.x] = INT_MAX;
+        bmax[threadIdx.x] = INT_MIN;
+    }
+    __syncthreads();
+    bool live = i < n && in.active[i];
+    if (i < n && !live) {
+        for (int a = 0; a < 3; ++a) {
+            lp.x[a][i] = 0.0;
+            lp.v[a][i] = 0.0;
+        }
+    }
+    const int64_t ii = live ? i : 0;
+    const int s = species_of(ii, sp.slots, sp.count);
+    const SpeciesK k = species_constants(d, sp, s);
+    const double h = sp.charge[s] * d.dt / (2.0 * sp.mass[s]);
+    double x[3], v[3], lx[3], lv[3];
+    for (int a = 0; a < 3; ++a) {
+        x[a] = live ? (double)in.x[a][ii] : 0.0;
+        v[a] = live ? (double)in.u[a][ii] : 0.0;
+        lx[a] = live ? lp.x[a][ii] : 0.0;   // adjoint of x' (the wrap is the identity map's derivative)
+        lv[a] = live ? lp.v[a][ii] : 0.0;   // adjoint of v'
+    }
+    // ---- phase 1
+    double Ep[3], Bp[3];
+    gather_values(d, fp, x, Ep, Bp);
+    double vp[3];
+    {
+        double vn[3];
+        push_t(v, Ep, Bp, h, d.pusher, d.relativistic, d.C, vn);
+        for (int a = 0; a < 3; ++a) vp[a] = ((k.uu >> a) & 1u) ? vn[a] : v[a];
+    }
+    for (int a = 0; a < 3; ++a)
+        if ((k.ux >> a) & 1u) lv[a] += d.dt * lx[a];   // x' = x + v' dt
+    {
+        double xnp[3];
+        for (int a = 0; a < 3; ++a) xnp[a] = ((k.ux >> a) & 1u) ? step_pos(x[a], vp[a], d.dt) : x[a];
+        double go[3] = {0, 0, 0}, gn[3] = {0, 0, 0}, gvd[3] = {0, 0, 0};
+        if constexpr (kAnalytic) {
+            if (live && !cic_esirkepov_adjoint(d, k.qw, x, xnp, vp, lp.J, go, gn, gvd)) {
+                defer[atomicAdd(defer_count, 1)] = (int32_t)i;
+                live = false;           // finished by the general pass
+            }
+            for (int a = 0; a < 3; ++a) {
+                lx[a] += go[a] + gn[a];
+                if ((k.ux >> a) & 1u) lv[a] += d.dt * gn[a];
+                lv[a] += gvd[a];
+            }
+        } else if (live) {
+            D6 xo[3], xnew[3], vd[3];
+            for (int a = 0; a < 3; ++a) {
+                xo[a] = dvar<6>(x[a], a);
+                vd[a] = dvar<6>(vp[a], 3 + a);
+                if ((k.ux >> a) & 1u) {
+                    xnew[a] = xo[a] + vd[a] * d.dt;
+                    xnew[a].v = xnp[a];    // same rounding as the forward anchor
+                } else {
+                    xnew[a] = xo[a];
+                }
+            }
+            double g6[6] = {0, 0, 0, 0, 0, 0};
+            esirkepov_adjoint(d, k.qw, xo, xnew, vd, lp.J, g6);
+            for (int a = 0; a < 3; ++a) {
+                lx[a] += g6[a];
+                lv[a] += g6[3 + a];
+            }
+        }
+    }
+    // ---- phase 2: lambda(v, E_p, B_p) = J_push^T lambda(v') on update_u axes
+    double lin[9];
+#pragma unroll 1
+    for (int m = 0; m < 9; ++m) {
+        Dual<1> vv[3], EE[3], BB[3], out[3];
+        for (int a = 0; a < 3; ++a) {
+            vv[a] = dconst<1>(v[a]);
+            EE[a] = dconst<1>(Ep[a]);
+            BB[a] = dconst<1>(Bp[a]);
+        }
+        if (m < 3) vv[m].t[0] = 1.0;
+        else if (m < 6) EE[m - 3].t[0] = 1.0;
+        else BB[m - 6].t[0] = 1.0;
+        push_t(vv, EE, BB, h, d.pusher, d.relativistic, d.C, out);
+        double acc = 0.0;
+        for (int a = 0; a < 3; ++a) acc += ((k.uu >> a) & 1u) ? lv[a] * out[a].t[0] : 0.0;
+        if (m < 3 && !((k.uu >> m) & 1u)) acc += lv[m];   // frozen component: v' = v
+        lin[m] = acc;
+    }
+    // ---- phase 3: gather stencils, dE_p/dx, scatter box
+    AdjGrid gr;
+    for (int a = 0; a < 3; ++a) {
+        bool act = d.n[a] > 1;
+        adj_gather_axis(x[a], d.org_c[a], d.sp_c[a], d.shape_factor, d.dims[a], act, d.g, gr.c[a]);
+        adj_gather_axis(x[a], d.org_v[a], d.sp_v[a], d.shape_factor, d.dims[a], act, d.g, gr.v[a]);
+    }
+    if (live) {
+        for (int a = 0; a < 3; ++a) {
+            const int lo = min(gr.c[a].r0, gr.v[a].r0);
+            const int hi = max(gr.c[a].r0 + gr.c[a].n, gr.v[a].r0 + gr.v[a].n) - 1;
+            atomicMin(&bmin[a], lo);
+            atomicMax(&bmax[a], hi);
+        }
+    }
+    __syncthreads();
+    ScatterBox sb;
+    sb.box = sbox;
+    sb.on = bmax[0] >= bmin[0];
+    int64_t vol64 = 1;
+    for (int a = 0; a < 3; ++a) {
+        sb.o[a] = bmin[a];
+        sb.e[a] = sb.on ? (int)min((int64_t)bmax[a] - bmin[a] + 1, (int64_t)kCap + 1) : 1;
+        vol64 *= sb.e[a];
+    }
+    sb.on = sb.on && 6 * vol64 <= kCap;
+    const int vol = sb.on ? (int)vol64 : 1;
+    if (sb.on)
+        for (int e = threadIdx.x; e < 6 * vol; e += blockDim.x) sbox[e] = 0.0;
+    __syncthreads();
+    if (live) {
+        for (int c = 0; c < 3; ++c) {
+            double val, g3[3];
+            gather_grad(d, gr, kStagE[c], fp.E[c], fp.Ee[c], val, g3);
+            for (int a = 0; a < 3; ++a) lx[a] += lin[3 + c] * g3[a];
+            gather_grad(d, gr, kStagB[c], fp.B[c], fp.Be[c], val, g3);
+            for (int a = 0; a < 3; ++a) lx[a] += lin[6 + c] * g3[a];
+            scatter_adj(d, gr, kStagE[c], lin[3 + c], lp.E[c], sb, c);
+            scatter_adj(d, gr, kStagB[c], lin[6 + c], lp.B[c], sb, 3 + c);
+        }
+        for (int a = 0; a < 3; ++a) {
+            lp.x[a][i] = lx[a];
+            lp.v[a][i] = lin[a];
+        }
+    }
+    __syncthreads();
+    if (sb.on) {
+        for (int e = threadIdx.x; e < 6 * vol; e += blockDim.x) {
+            const double val = sbox[e];
+            if (val == 0.0) continue;
+            const int c = e / vol, r = e % vol;
+            const int bi = r / (sb.e[1] * sb.e[2]), bj = (r / sb.e[2]) % sb.e[1], bk = r % sb.e[2];
+            double* G = c < 3 ? lp.E[c] : lp.B[c - 3];
+            unsafeAtomicAdd(&G[partner_index(d, sb.o[0] + bi, sb.o[1] + bj, sb.o[2] + bk)], val);
+        }
+    }
+    __syncthreads();   // the next pass reuses the box
+    }
+}
+
+__global__ void zero_count_kernel(int32_t* c) { *c = 0; }
 
 // ------------------------------------------------------------ field kernels
 // Interior arrays (Nx, Ny, Nz), z fastest, periodic wrap.
@@ -717,12 +971,14 @@ int pic_adjoint_yee(const PicDesc* desc, double* const lamE[3], double* const la
 int pic_adjoint_particles(const PicDesc* desc, const PicSpecies* sp, const PicParticles* state,
                           const float* const E[3], const float* const B[3], const float* const E_ext[3],
                           const float* const B_ext[3], const double* const lamJ[3], double* const lam_x[3],
-                          double* const lam_v[3], double* const lamE[3], double* const lamB[3], void* stream) {
+                          double* const lam_v[3], double* const lamE[3], double* const lamB[3], int32_t min_blocks,
+                          int32_t* workspace, void* stream) {
     PIC_REQUIRE(desc_ok(desc) && periodic_all(desc) && particles_ok(sp, state) && E && B && lamJ && lam_x &&
-                lam_v && lamE && lamB);
+                lam_v && lamE && lamB && workspace);
     for (int a = 0; a < 3; ++a)
         PIC_REQUIRE(E[a] && B[a] && lamJ[a] && lam_x[a] && lam_v[a] && lamE[a] && lamB[a]);
     int64_t n = (int64_t)sp->count * sp->slots;
+    PIC_REQUIRE(n < ((int64_t)1 << 31));
     if (n == 0) return 0;
     FieldPtrs fp = make_fields(E, B, E_ext, B_ext);
     LamPtrs lp;
@@ -733,8 +989,29 @@ int pic_adjoint_particles(const PicDesc* desc, const PicSpecies* sp, const PicPa
         lp.E[a] = lamE[a];
         lp.B[a] = lamB[a];
     }
-    hipLaunchKernelGGL(adjoint_particles_kernel, dim3(grid_1d(n, 128)), dim3(128), 0, as_stream(stream), *desc,
-                       *sp, *state, fp, lp, n);
+    int32_t* count = workspace;
+    int32_t* list = workspace + 1;
+    const dim3 grid(grid_1d(n, kAdjThreads)), blk(kAdjThreads);
+    const dim3 grid2(grid_1d(n, kAdjThreads) < 512 ? grid_1d(n, kAdjThreads) : 512);
+    hipStream_t st = as_stream(stream);
+    if (desc->shape_factor != 1) {
+        hipLaunchKernelGGL((adjoint_particles_kernel<false, 1>), grid, blk, 0, st, *desc, *sp, *state, fp, lp, n,
+                           nullptr, nullptr, nullptr, nullptr);
+        PIC_CHECK_LAUNCH();
+        return 0;
+    }
+    hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, count);
+    if (min_blocks >= 8)
+        hipLaunchKernelGGL((adjoint_particles_kernel<true, 8>), grid, blk, 0, st, *desc, *sp, *state, fp, lp, n, list,
+                           count, nullptr, nullptr);
+    else if (min_blocks >= 4)
+        hipLaunchKernelGGL((adjoint_particles_kernel<true, 4>), grid, blk, 0, st, *desc, *sp, *state, fp, lp, n, list,
+                           count, nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((adjoint_particles_kernel<true, 1>), grid, blk, 0, st, *desc, *sp, *state, fp, lp, n, list,
+                           count, nullptr, nullptr);
+    hipLaunchKernelGGL((adjoint_particles_kernel<false, 1>), grid2, blk, 0, st, *desc, *sp, *state, fp, lp, n, nullptr,
+                       nullptr, list, count);
     PIC_CHECK_LAUNCH();
     return 0;
 }
