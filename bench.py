@@ -81,14 +81,15 @@ def extras(n, ppc, steps, seed):
         static, dyn, species = workloads.config_warm_3d(n=n, ppc=ppc, vth=0.05, current_deposition="j_from_rhov")
         static = static._replace(current_filter=filt)
         parts, cfg = workloads.plasma(static, dyn, species, seed=seed)
-        r = ElectrodynamicRunner(parts, cfg, workloads.empty_fields(static, dyn), static, dyn, sort_every=10)
+        r = ElectrodynamicRunner(parts, cfg, workloads.empty_fields(static, dyn), static, dyn, sort_every=10, tile=8)
         del parts
         ms = _timed(r.step, steps)
         npart = r.S * r.P
         out[f"j_from_rhov_{filt}"] = {"ms_per_step": ms, "particle_steps_per_s": npart / (ms * 1e-3)}
         if filt == "none":
             rho = torch.empty_like(r.J[0])
-            ms_rho = _timed(lambda: deposit_rho_into(r.desc, r.sp, r._pp(r.cur), rho, "none", N.stream_handle()), 3)
+            ms_rho = _timed(lambda: deposit_rho_into(r.desc, r.sp, r._pp(r.cur), rho, "none", N.stream_handle(),
+                                                     tile=r.tile, starts=r.starts), 3)
             out["rho"] = {"ms": ms_rho, "particles_per_s": npart / (ms_rho * 1e-3),
                           "algorithmic_bytes": 13 * npart + 4 * n_cells}
             phi = torch.zeros_like(rho)

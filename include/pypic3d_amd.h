@@ -128,6 +128,14 @@ int pic_deposit_direct(const PicDesc* desc, const PicSpecies* sp, const PicParti
                        float* const J[3], void* stream);
 int pic_deposit_rho(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
                     float* rho, void* stream);
+/* Tile-ordered versions of the two deposits above for particles sorted by
+ * pic_tile_keys (ranges from pic_tile_ranges, tile 4 or 8): one block per
+ * (species, tile) accumulates in an LDS box around the tile and adds it into
+ * the arrays once.  Same results up to FP32 summation order. */
+int pic_deposit_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, float* const J[3],
+                             int32_t tile, const int32_t* starts, void* stream);
+int pic_deposit_rho_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, float* rho, int32_t tile,
+                          const int32_t* starts, void* stream);
 /* Fused particle phase of time_loop_electrodynamic's Esirkepov branch
  * (evolve.py:36-43, 70-79): gather + push, raw Esirkepov deposit into J (zeroed
  * by the caller), x += v dt, particle boundaries.  Writes out (must not alias

@@ -72,6 +72,8 @@ SIGNATURES = {
     "pic_deposit_esirkepov": (c_int, [DP, SP, PP, P3, c_vp]),
     "pic_deposit_direct": (c_int, [DP, SP, PP, P3, c_vp]),
     "pic_deposit_rho": (c_int, [DP, SP, PP, c_vp, c_vp]),
+    "pic_deposit_direct_tiled": (c_int, [DP, SP, PP, P3, c_i32, c_vp, c_vp]),
+    "pic_deposit_rho_tiled": (c_int, [DP, SP, PP, c_vp, c_i32, c_vp, c_vp]),
     "pic_fused_push_esirkepov": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, PP, P3, c_vp]),
     "pic_fused_push_esirkepov_tiled": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, PP, P3, c_i32, c_vp, c_i32, c_i32,
                                                c_vp]),

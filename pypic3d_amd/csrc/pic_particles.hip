@@ -89,43 +89,135 @@ __global__ void esirkepov_kernel(PicDesc d, PicSpecies sp, PicParticles in, floa
     esirkepov_particle(d, sp.charge[s] * sp.weight[s], xo, xn, v, J);
 }
 
-__global__ void direct_kernel(PicDesc d, PicSpecies sp, PicParticles in, float* Jx, float* Jy, float* Jz, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (!in.active[i]) return;
-    int s = species_of(i, sp.slots, sp.count);
+// J_from_rhov deposit of particle i (J_from_rhov.py:88-200) into a sink.
+template <class Sink>
+__device__ inline void direct_particle(const PicDesc& d, float dqv, const PicParticles& in, int64_t i,
+                                       const Sink& sink) {
     double pos[3];
     load_pos(in, i, pos);
     NodeAxis ax[3];
     for (int a = 0; a < 3; ++a)
         node_axis(pos[a], d.org_c[a], d.sp_c[a], d.d[a], d.shape_factor, d.n[a] == 1, d.g, ax[a]);
-    float dq = (float)(sp.charge[s] * sp.weight[s] / (d.d[0] * d.d[1] * d.d[2]));
-    float q[3] = {dq * in.u[0][i], dq * in.u[1][i], dq * in.u[2][i]};
-    for (int a = ax[0].k0; a < ax[0].k1; ++a)
-        for (int b = ax[1].k0; b < ax[1].k1; ++b)
-            for (int c = ax[2].k0; c < ax[2].k1; ++c) {
-                int ix = ax[0].p[a], iy = ax[1].p[b], iz = ax[2].p[c];
-                deposit_at(Jx, d, ix, iy, iz, q[0] * ax[0].wf[a] * ax[1].wn[b] * ax[2].wn[c]);
-                deposit_at(Jy, d, ix, iy, iz, q[1] * ax[0].wn[a] * ax[1].wf[b] * ax[2].wn[c]);
-                deposit_at(Jz, d, ix, iy, iz, q[2] * ax[0].wn[a] * ax[1].wn[b] * ax[2].wf[c]);
+    const float q[3] = {dqv * in.u[0][i], dqv * in.u[1][i], dqv * in.u[2][i]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (a < ax[0].k0 || a >= ax[0].k1 || b < ax[1].k0 || b >= ax[1].k1 || c < ax[2].k0 || c >= ax[2].k1)
+                    continue;
+                const int ix = ax[0].p[a], iy = ax[1].p[b], iz = ax[2].p[c];
+                sink.add(d, 0, ix, iy, iz, q[0] * ax[0].wf[a] * ax[1].wn[b] * ax[2].wn[c]);
+                sink.add(d, 1, ix, iy, iz, q[1] * ax[0].wn[a] * ax[1].wf[b] * ax[2].wn[c]);
+                sink.add(d, 2, ix, iy, iz, q[2] * ax[0].wn[a] * ax[1].wn[b] * ax[2].wf[c]);
             }
+}
+
+// compute_rho deposit of particle i (rho.py:72-153) into component 0 of a sink.
+template <class Sink>
+__device__ inline void rho_particle(const PicDesc& d, float qd, const PicParticles& in, int64_t i, const Sink& sink) {
+    double pos[3];
+    load_pos(in, i, pos);
+    NodeAxis ax[3];
+    for (int a = 0; a < 3; ++a)
+        node_axis(pos[a], d.org_c[a], d.sp_c[a], d.d[a], d.shape_factor, d.n[a] == 1, d.g, ax[a]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (a < ax[0].k0 || a >= ax[0].k1 || b < ax[1].k0 || b >= ax[1].k1 || c < ax[2].k0 || c >= ax[2].k1)
+                    continue;
+                sink.add(d, 0, ax[0].p[a], ax[1].p[b], ax[2].p[c], qd * ax[0].wn[a] * ax[1].wn[b] * ax[2].wn[c]);
+            }
+}
+
+__device__ inline float charge_density(const PicDesc& d, const PicSpecies& sp, int s) {
+    return (float)(sp.charge[s] * sp.weight[s] / (d.d[0] * d.d[1] * d.d[2]));
+}
+
+// Deposit sinks with plain global atomics (slot-ordered launches).
+struct GlobalDeposit {
+    float* J[3];
+    __device__ inline void add(const PicDesc& d, int c, int i, int j, int k, float v) const {
+        deposit_at(J[c], d, i, j, k, v);
+    }
+};
+
+__global__ void direct_kernel(PicDesc d, PicSpecies sp, PicParticles in, float* Jx, float* Jy, float* Jz, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!in.active[i]) return;
+    const GlobalDeposit sink{{Jx, Jy, Jz}};
+    direct_particle(d, charge_density(d, sp, species_of(i, sp.slots, sp.count)), in, i, sink);
 }
 
 __global__ void rho_kernel(PicDesc d, PicSpecies sp, PicParticles in, float* rho, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (!in.active[i]) return;
-    int s = species_of(i, sp.slots, sp.count);
-    double pos[3];
-    load_pos(in, i, pos);
-    NodeAxis ax[3];
-    for (int a = 0; a < 3; ++a)
-        node_axis(pos[a], d.org_c[a], d.sp_c[a], d.d[a], d.shape_factor, d.n[a] == 1, d.g, ax[a]);
-    float qd = (float)(sp.charge[s] * sp.weight[s] / (d.d[0] * d.d[1] * d.d[2]));
-    for (int a = ax[0].k0; a < ax[0].k1; ++a)
-        for (int b = ax[1].k0; b < ax[1].k1; ++b)
-            for (int c = ax[2].k0; c < ax[2].k1; ++c)
-                deposit_at(rho, d, ax[0].p[a], ax[1].p[b], ax[2].p[c], qd * ax[0].wn[a] * ax[1].wn[b] * ax[2].wn[c]);
+    const GlobalDeposit sink{{rho, rho, rho}};
+    rho_particle(d, charge_density(d, sp, species_of(i, sp.slots, sp.count)), in, i, sink);
+}
+
+// LDS box sink for the tile-ordered deposits: kC components of B^3 nodes from
+// origin o (array indices); nodes outside go straight to the global arrays.
+template <int kC, int kB>
+struct DepositBox {
+    float* J[3];
+    float* box;
+    int o[3];
+    __device__ inline void add(const PicDesc& d, int c, int i, int j, int k, float v) const {
+        if (v == 0.0f) return;
+        const int bi = i - o[0], bj = j - o[1], bk = k - o[2];
+        if ((unsigned)bi < (unsigned)kB && (unsigned)bj < (unsigned)kB && (unsigned)bk < (unsigned)kB)
+            atomicAdd(&box[((c * kB + bi) * kB + bj) * kB + bk], v);
+        else
+            deposit_at(J[c], d, i, j, k, v);
+    }
+};
+
+// Tile-ordered deposits (particles sorted by pic_tile_keys): block b owns the
+// slots [starts[b], starts[b+1]) of one (species, tile); deposits accumulate in
+// an LDS box of (T+5)^3 nodes from T*tile + g - 2 and are added into the
+// arrays once per block.
+template <int kC, int kT, bool kRho>
+__global__ __launch_bounds__(kBlock) void tiled_deposit_kernel(PicDesc d, PicSpecies sp, PicParticles in, float* J0,
+                                                               float* J1, float* J2,
+                                                               const int32_t* __restrict__ starts, int tiles,
+                                                               int nt1, int nt2) {
+    constexpr int kB = kT + 5;
+    constexpr int kV = kB * kB * kB;
+    __shared__ float box[kC * kV];
+    const int b = blockIdx.x;
+    const int64_t begin = starts[b], end = starts[b + 1];
+    if (begin >= end) return;
+    const int t = b % (tiles + 1);
+    int o[3] = {0, 0, 0};
+    if (t < tiles) {
+        o[0] = (t / (nt1 * nt2)) * kT + d.g - 2;
+        o[1] = ((t / nt2) % nt1) * kT + d.g - 2;
+        o[2] = (t % nt2) * kT + d.g - 2;
+    }
+    for (int e = threadIdx.x; e < kC * kV; e += kBlock) box[e] = 0.0f;
+    __syncthreads();
+    const DepositBox<kC, kB> sink{{J0, J1, J2}, box, {o[0], o[1], o[2]}};
+    for (int64_t i = begin + threadIdx.x; i < end; i += kBlock) {
+        if (!in.active[i]) continue;
+        const float q = charge_density(d, sp, species_of(i, sp.slots, sp.count));
+        if constexpr (kRho) rho_particle(d, q, in, i, sink);
+        else direct_particle(d, q, in, i, sink);
+    }
+    __syncthreads();
+    float* const J[3] = {J0, J1, J2};
+    for (int e = threadIdx.x; e < kC * kV; e += kBlock) {
+        const float v = box[e];
+        if (v == 0.0f) continue;
+        const int c = e / kV, r = e % kV;
+        deposit_at(J[c], d, o[0] + r / (kB * kB), o[1] + (r / kB) % kB, o[2] + r % kB, v);
+    }
 }
 
 }  // namespace
@@ -186,6 +278,45 @@ int pic_deposit_direct(const PicDesc* desc, const PicSpecies* sp, const PicParti
                        J[0], J[1], J[2], n);
     PIC_CHECK_LAUNCH();
     return 0;
+}
+
+static int tiled_deposit(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, float* const J[3],
+                         int32_t tile, const int32_t* starts, bool rho, void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && particles_ok(sp, in) && J && J[0] && starts && (tile == 4 || tile == 8));
+    if (!rho) PIC_REQUIRE(J[1] && J[2]);
+    int nt[3];
+    for (int a = 0; a < 3; ++a) nt[a] = (desc->n[a] + tile - 1) / tile;
+    const int64_t tiles = (int64_t)nt[0] * nt[1] * nt[2];
+    const int64_t blocks = (int64_t)sp->count * (tiles + 1);
+    PIC_REQUIRE(blocks < ((int64_t)1 << 31));
+    if ((int64_t)sp->count * sp->slots == 0) return 0;
+    hipStream_t st = as_stream(stream);
+    float* j1 = rho ? J[0] : J[1];
+    float* j2 = rho ? J[0] : J[2];
+#define PIC_TD(C, T, R)                                                                                           \
+    hipLaunchKernelGGL((tiled_deposit_kernel<C, T, R>), dim3((unsigned)blocks), dim3(kBlock), 0, st, *desc, *sp, *in, \
+                       J[0], j1, j2, starts, (int)tiles, nt[1], nt[2])
+    if (rho) {
+        if (tile == 8) PIC_TD(1, 8, true);
+        else PIC_TD(1, 4, true);
+    } else {
+        if (tile == 8) PIC_TD(3, 8, false);
+        else PIC_TD(3, 4, false);
+    }
+#undef PIC_TD
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+int pic_deposit_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, float* const J[3],
+                             int32_t tile, const int32_t* starts, void* stream) {
+    return tiled_deposit(desc, sp, in, J, tile, starts, false, stream);
+}
+
+int pic_deposit_rho_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, float* rho, int32_t tile,
+                          const int32_t* starts, void* stream) {
+    float* const J[3] = {rho, rho, rho};
+    return tiled_deposit(desc, sp, in, J, tile, starts, true, stream);
 }
 
 int pic_deposit_rho(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, float* rho, void* stream) {

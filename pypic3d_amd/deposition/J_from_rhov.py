@@ -12,10 +12,14 @@ from .. import _native as N
 from .._tensors import as_field3, check_fields, like_input, make_desc, make_species, particle_planes, pic_particles
 
 
-def deposit_direct_into(desc, sp, pp, J3, current_filter, stream, scratch=None):
+def deposit_direct_into(desc, sp, pp, J3, current_filter, stream, scratch=None, tile=None, starts=None):
+    """tile/starts: tile-ordered deposit for particles sorted by pic_tile_keys."""
     for c in J3:
         N.call("pic_fill_f32", N.ptr(c), c.numel(), 0.0, stream)
-    N.call("pic_deposit_direct", desc, sp, pp, N.p3(J3), stream)
+    if tile:
+        N.call("pic_deposit_direct_tiled", desc, sp, pp, N.p3(J3), int(tile), N.ptr(starts), stream)
+    else:
+        N.call("pic_deposit_direct", desc, sp, pp, N.p3(J3), stream)
     for c in J3:
         N.call("pic_halo_fold", desc, N.ptr(c), 1, stream)
         N.call("pic_halo_refresh", desc, N.ptr(c), 1, stream)

@@ -10,9 +10,13 @@ from .. import _native as N
 from .._tensors import as_field3, check_fields, like_input, make_desc, make_species, particle_planes, pic_particles
 
 
-def deposit_rho_into(desc, sp, pp, rho3, current_filter, stream, scratch=None):
+def deposit_rho_into(desc, sp, pp, rho3, current_filter, stream, scratch=None, tile=None, starts=None):
+    """tile/starts: tile-ordered deposit for particles sorted by pic_tile_keys."""
     N.call("pic_fill_f32", N.ptr(rho3), rho3.numel(), 0.0, stream)
-    N.call("pic_deposit_rho", desc, sp, pp, N.ptr(rho3), stream)
+    if tile:
+        N.call("pic_deposit_rho_tiled", desc, sp, pp, N.ptr(rho3), int(tile), N.ptr(starts), stream)
+    else:
+        N.call("pic_deposit_rho", desc, sp, pp, N.ptr(rho3), stream)
     N.call("pic_halo_fold", desc, N.ptr(rho3), 1, stream)
     N.call("pic_halo_refresh", desc, N.ptr(rho3), 1, stream)
     if current_filter == "digital":

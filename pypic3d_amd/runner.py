@@ -28,9 +28,10 @@ class ElectrodynamicRunner:
 
     def __init__(self, particles, species_config, fields, static_parameters, dynamic_parameters, sort_every=0,
                  tile=None, rounds=4, min_blocks=1):
-        """tile (4 or 8): use the tile-ordered fused kernel (3-D first-order
-        Esirkepov only) with particles kept in tile-major cell order; the
-        particles are sorted once here and then every ``sort_every`` steps."""
+        """tile (4 or 8): keep particles in tile-major cell order (sorted once
+        here, then every ``sort_every`` steps) and use the tile-ordered kernels:
+        the fused push + Esirkepov kernel (3-D first order) or the direct
+        deposit of the J_from_rhov branch."""
         E, B, J, rho, phi, external_fields, pml_state, overflow = fields
         if pml_state is not None:
             raise NotImplementedError("PML is not implemented on the GPU path")
@@ -131,7 +132,8 @@ class ElectrodynamicRunner:
             if ev:
                 ev[1].record()
             ev = self._mark("current_fold")
-            deposit_direct_into(d, self.sp, self._pp(dst), self.J, self.filter, s, scratch=self.scratch)
+            deposit_direct_into(d, self.sp, self._pp(dst), self.J, self.filter, s, scratch=self.scratch,
+                                tile=self.tile, starts=self.starts)
             N.call("pic_advance_positions", d, self.sp, self._pp(dst), 0.5, N.p3(xd), s)
             N.call("pic_particle_boundaries", d, self.sp, self._pp(dst), s)
         if ev:
@@ -185,8 +187,8 @@ class ElectrodynamicRunner:
             self.tile, self.starts = None, None
             return
         d = self.desc
-        if not (self.esirkepov and d.shape_factor == 1 and all(d.n[a] > 1 for a in range(3))):
-            raise ValueError("the tile-ordered kernel needs the 3-D first-order Esirkepov path")
+        if self.esirkepov and not (d.shape_factor == 1 and all(d.n[a] > 1 for a in range(3))):
+            raise ValueError("the tile-ordered Esirkepov kernel needs the 3-D first-order path")
         if tile not in (4, 8):
             raise ValueError("tile must be 4 or 8")
         self.tile = int(tile)

@@ -180,12 +180,16 @@ def test_fused_step_on_cell_sorted_dense_plasma_matches_oracle(N, ppc, pusher):
         assert rel_err(host(fields[1][c]), want.B[c]) <= TOL, ("B", c)
 
 
-@pytest.mark.parametrize("tile,rounds,min_blocks", [(None, 2, 1), (8, 2, 3), (4, 2, 1), (8, 4, 1), (4, 4, 4)])
-def test_runner_matches_time_loop_and_sorting_is_transparent(tile, rounds, min_blocks):
+@pytest.mark.parametrize("tile,rounds,min_blocks,dep", [(None, 2, 1, "esirkepov"), (8, 2, 3, "esirkepov"),
+                                                       (4, 2, 1, "esirkepov"), (8, 4, 1, "esirkepov"),
+                                                       (4, 4, 4, "esirkepov"), (8, 4, 1, "direct"),
+                                                       (4, 4, 1, "direct")])
+def test_runner_matches_time_loop_and_sorting_is_transparent(tile, rounds, min_blocks, dep):
     """Slot-ordered and tile-ordered runners (re-sorting every 2 steps, the
     tile grid not dividing the domain) against the functional step."""
     from pypic3d_amd.runner import ElectrodynamicRunner
-    static, dyn = workloads.make_parameters((16, 12, 10), cfl=0.5)
+    static, dyn = workloads.make_parameters((16, 12, 10), cfl=0.5, current_deposition=dep,
+                                            current_filter="bilinear" if dep == "direct" else "none")
     specs = [workloads.SpeciesSpec(16, weight=0.01, vth=(0.1,) * 3),
              workloads.SpeciesSpec(5, charge=1.0, mass=20.0, weight=0.03, vth=(0.02,) * 3)]
     parts, cfg = workloads.plasma(static, dyn, specs, seed=5)
@@ -212,3 +216,24 @@ def test_runner_matches_time_loop_and_sorting_is_transparent(tile, rounds, min_b
     for k in (0, 1, 2):
         for c in range(3):
             assert rel_err(host(r_fields[k][c]), host(ref_fields[k][c])) <= 1e-5, (k, c)
+
+
+@pytest.mark.parametrize("tile,sf", [(8, 1), (4, 1), (4, 2)])
+def test_tile_ordered_rho_matches_oracle(tile, sf):
+    """compute_rho's tile-ordered deposit on tile-sorted particles."""
+    from pypic3d_amd import _native as N
+    from pypic3d_amd.deposition.rho import deposit_rho_into
+    from pypic3d_amd.runner import ElectrodynamicRunner
+    static, dyn = workloads.make_parameters((14, 9, 11), cfl=0.5, shape_factor=sf, current_deposition="direct")
+    specs = [workloads.SpeciesSpec(6, weight=0.02, vth=(0.1,) * 3),
+             workloads.SpeciesSpec(3, charge=1.0, mass=30.0, weight=0.05, vth=(0.02,) * 3)]
+    parts, cfg = workloads.plasma(static, dyn, specs, seed=9)
+    r = ElectrodynamicRunner(parts, cfg, workloads.empty_fields(static, dyn), static, dyn, tile=tile)
+    r.step()             # drift off the sorted cells before depositing
+    rho = torch.empty_like(r.J[0])
+    deposit_rho_into(r.desc, r.sp, r._pp(r.cur), rho, "none", N.stream_handle(), tile=r.tile, starts=r.starts)
+    p = oracle_params(static, dyn)
+    x, u, act = host_particles(r.state()[0])
+    from tests.gpu_helpers import oracle_species
+    want = O.compute_rho(x, act, oracle_species(cfg), p)
+    assert rel_err(host(rho), want) <= TOL
