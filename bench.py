@@ -133,7 +133,6 @@ def ensemble(n_members, steps, seed):
     del batch
     torch.cuda.empty_cache()
     singles = [ElectrodynamicRunner(*m) for m in members]
-    del members
     for r in singles:
         r.step()
     torch.cuda.synchronize()
@@ -144,31 +143,46 @@ def ensemble(n_members, steps, seed):
     b.record()
     torch.cuda.synchronize()
     t_seq = a.elapsed_time(b) / steps
-    # the same sequential launches recorded in one graph (no host launch cost)
-    g = torch.cuda.CUDAGraph()
-    s = torch.cuda.Stream()
-    s.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(s):
-        for r in singles:
-            r.step()
-    torch.cuda.current_stream().wait_stream(s)
-    torch.cuda.synchronize()
-    with torch.cuda.graph(g):
-        for r in singles:       # two steps per member: the replay advances the state
-            r.step()
-            r.step()
-    g.replay()
-    torch.cuda.synchronize()
-    a.record()
-    for _ in range(reps):
+
+    def graph_of(runners):
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            for r in runners:
+                r.step()
+                r.step()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g):
+            for r in runners:       # two steps per member: the replay advances the state
+                r.step()
+                r.step()
         g.replay()
-    b.record()
-    torch.cuda.synchronize()
-    t_seq_graph = a.elapsed_time(b) / (2 * reps)
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(reps):
+            g.replay()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / (2 * reps)
+
+    # the same sequential launches recorded in one graph (no host launch cost)
+    t_seq_graph = graph_of(singles)
+    del singles
+    torch.cuda.empty_cache()
+    # the batched kernels launched for one member at a time, also in one graph
+    solo = [EnsembleRunner([m], rounds=rounds, grid=grid) for m in members]
+    del members
+    t_solo_graph = graph_of(solo)
+    del solo
+    torch.cuda.empty_cache()
     return {"members": n_members, "particles": npart, "rounds": rounds, "grid": grid,
             "batched_ms_per_step": t_batch, "sequential_ms_per_step": t_seq,
             "sequential_graph_ms_per_step": t_seq_graph,
+            "batched_kernels_one_member_at_a_time_graph_ms_per_step": t_solo_graph,
             "speedup_vs_sequential": t_seq / t_batch, "speedup_vs_sequential_graph": t_seq_graph / t_batch,
+            "speedup_vs_batched_kernels_one_member_at_a_time": t_solo_graph / t_batch,
             "batched_particle_steps_per_s": npart / (t_batch * 1e-3)}
 
 

@@ -58,11 +58,15 @@ struct BatchParticleArgs {
     float* J[3];
 };
 
+// Each work item runs fused_block_boxed (J accumulated in an LDS box sized to
+// the item's particles).
 template <int kRounds>
 __global__ __launch_bounds__(kThreads, 1) void batch_fused_kernel(const PicBatchSim* __restrict__ sims,
                                                                   const int32_t* __restrict__ work, int32_t n_work,
                                                                   BatchParticleArgs args) {
     __shared__ FusedShared<kRounds> sh;
+    __shared__ float box[kRangeBoxCap];
+    __shared__ int bmin[3], bmax[3];
     for (int32_t w = blockIdx.x; w < n_work; w += gridDim.x) {
         const int sim = work[2 * w];
         const uint32_t blk = (uint32_t)work[2 * w + 1];
@@ -73,15 +77,10 @@ __global__ __launch_bounds__(kThreads, 1) void batch_fused_kernel(const PicBatch
         const PicParticles in = offset_particles(args.in, poff);
         const PicParticles out = offset_particles(args.out, poff);
         const FieldPtrs fp = batch_field_ptrs(args.E, args.B, foff);
-        const GlobalSink sink{{args.J[0] + foff, args.J[1] + foff, args.J[2] + foff}};
+        float* const J[3] = {args.J[0] + foff, args.J[1] + foff, args.J[2] + foff};
         const uint32_t base = blk * (uint32_t)(kThreads * kRounds);
-        switch (fused_kind(d)) {
-            case 3: fused_block<3, kRounds>(d, bs.species, in, fp, out, sink, n, base, sh); break;
-            case 2: fused_block<2, kRounds>(d, bs.species, in, fp, out, sink, n, base, sh); break;
-            case 1: fused_block<1, kRounds>(d, bs.species, in, fp, out, sink, n, base, sh); break;
-            default: fused_block<0, kRounds>(d, bs.species, in, fp, out, sink, n, base, sh); break;
-        }
-        __syncthreads();   // the next item reuses the shared queue
+        fused_block_boxed<kRounds>(d, bs.species, in, fp, out, J, n, base, sh, box, bmin, bmax);
+        __syncthreads();   // the next item reuses the shared queue and box
     }
 }
 

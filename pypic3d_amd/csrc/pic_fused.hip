@@ -87,6 +87,20 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tiled_kernel(PicDe
     }
 }
 
+// Slot-ordered launch for 1-D and 2-D first-order runs (and the general
+// path): each block accumulates J in an LDS box sized to its particles.
+template <int kRounds>
+__global__ __launch_bounds__(kThreads, 1) void fused_boxed_kernel(PicDesc d, PicSpecies sp, PicParticles in,
+                                                                  FieldPtrs fp, PicParticles out, float* Jx,
+                                                                  float* Jy, float* Jz, int64_t n) {
+    __shared__ FusedShared<kRounds> sh;
+    __shared__ float box[kRangeBoxCap];
+    __shared__ int bmin[3], bmax[3];
+    float* const J[3] = {Jx, Jy, Jz};
+    fused_block_boxed<kRounds>(d, sp, in, fp, out, J, n, blockIdx.x * (uint32_t)(kThreads * kRounds), sh, box, bmin,
+                               bmax);
+}
+
 }  // namespace
 
 extern "C" {
@@ -118,12 +132,9 @@ int pic_fused_push_esirkepov(const PicDesc* desc, const PicSpecies* sp, const Pi
             case 5: PIC_FUSED_LAUNCH(3, 8, 3); break;
             default: PIC_FUSED_LAUNCH(3, 4, 1); break;
         }
-    } else if (kind == 2) {
-        PIC_FUSED_LAUNCH(2, 4, 1);
-    } else if (kind == 1) {
-        PIC_FUSED_LAUNCH(1, 4, 1);
     } else {
-        PIC_FUSED_LAUNCH(0, 4, 1);
+        hipLaunchKernelGGL(fused_boxed_kernel<4>, dim3(grid_1d(n, kThreads * 4)), dim3(kThreads), 0, st, *desc, *sp,
+                           *in, fp, *out, J[0], J[1], J[2], n);
     }
 #undef PIC_FUSED_LAUNCH
     PIC_CHECK_LAUNCH();

@@ -164,6 +164,24 @@ struct BoxSink {
     }
 };
 
+// LDS box with per-axis extents e[] from origin o[] (array indices), three
+// components; used when a block's particles cover a small, data-dependent
+// range of cells.  Nodes outside (or `on` false) go to the global arrays.
+struct RangeBoxSink {
+    float* J[3];
+    float* box;
+    int o[3], e[3];
+    bool on;
+    __device__ inline void add(const PicDesc& d, int c, int i, int j, int k, float v) const {
+        if (v == 0.0f) return;
+        const int bi = i - o[0], bj = j - o[1], bk = k - o[2];
+        if (on && (unsigned)bi < (unsigned)e[0] && (unsigned)bj < (unsigned)e[1] && (unsigned)bk < (unsigned)e[2])
+            atomicAdd(&box[((c * e[0] + bi) * e[1] + bj) * e[2] + bk], v);
+        else
+            deposit_at(J[c], d, i, j, k, v);
+    }
+};
+
 // First-order Esirkepov for a particle whose anchor moves by at most one cell
 // per axis, in a 3-slot frame starting at b = min(a_old, a_new): S0, D = S1 -
 // S0 (FP64) and the running sum C of D at slots 0 and 1 (slot 2's sum is 0).
@@ -686,6 +704,79 @@ __host__ __device__ inline int fused_kind(const PicDesc& d) {
     if (a0 && a1 && !a2) return 2;
     if (a0 && !a1 && !a2) return 1;
     return 0;
+}
+
+
+// The fused block with its J accumulated in an LDS box sized to the block's
+// particles: the block first bounds the center-grid anchors of its slots
+// [base, base + kThreads*kRounds) (FP32 estimate, padded by two nodes below
+// and three above for the rounding, the stencil and face crossings), then
+// runs fused_block into the box and adds the box into J once.  A block whose
+// box does not fit deposits straight into J.  `box` holds kRangeBoxCap floats.
+constexpr int kRangeBoxCap = 8192;
+
+template <int kRounds>
+__device__ inline void fused_block_boxed(const PicDesc& d, const PicSpecies& sp, const PicParticles& in,
+                                         const FieldPtrs& fp, const PicParticles& out, float* const J[3], int64_t n,
+                                         uint32_t base, FusedShared<kRounds>& sh, float* box, int* bmin, int* bmax) {
+    if (threadIdx.x < 3) {
+        bmin[threadIdx.x] = INT_MAX;
+        bmax[threadIdx.x] = INT_MIN;
+    }
+    __syncthreads();
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (int r = 0; r < kRounds; ++r) {
+        const uint32_t i = base + (uint32_t)(r * kThreads + threadIdx.x);
+        if (i < n && in.active[i]) {
+            for (int a = 0; a < 3; ++a) {
+                const float t = (in.x[a][i] - (float)d.org_c[a]) * (float)(1.0 / d.sp_c[a]);
+                const int c = (int)fminf(fmaxf(floorf(t), -1.0e6f), 1.0e6f);
+                lo[a] = min(lo[a], c);
+                hi[a] = max(hi[a], c);
+            }
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (lo[a] <= hi[a]) {
+            atomicMin(&bmin[a], lo[a]);
+            atomicMax(&bmax[a], hi[a]);
+        }
+    }
+    __syncthreads();
+    RangeBoxSink sink;
+    for (int c = 0; c < 3; ++c) sink.J[c] = J[c];
+    sink.box = box;
+    sink.on = bmax[0] >= bmin[0];
+    int64_t vol = 1;
+    for (int a = 0; a < 3; ++a) {
+        if (d.n[a] == 1) {
+            sink.o[a] = d.g;           // reduced axis: every node sits at index g
+            sink.e[a] = 1;
+        } else {
+            sink.o[a] = bmin[a] - 2;
+            sink.e[a] = (int)min((int64_t)bmax[a] - bmin[a] + 6, (int64_t)kRangeBoxCap);
+        }
+        vol *= sink.e[a];
+    }
+    sink.on = sink.on && 3 * vol <= kRangeBoxCap;
+    const int nbox = sink.on ? (int)(3 * vol) : 0;
+    for (int e = threadIdx.x; e < nbox; e += kThreads) box[e] = 0.0f;
+    __syncthreads();
+    switch (fused_kind(d)) {
+        case 3: fused_block<3, kRounds>(d, sp, in, fp, out, sink, n, base, sh); break;
+        case 2: fused_block<2, kRounds>(d, sp, in, fp, out, sink, n, base, sh); break;
+        case 1: fused_block<1, kRounds>(d, sp, in, fp, out, sink, n, base, sh); break;
+        default: fused_block<0, kRounds>(d, sp, in, fp, out, sink, n, base, sh); break;
+    }
+    __syncthreads();
+    const int v1 = sink.e[1] * sink.e[2], v3 = (int)vol;
+    for (int e = threadIdx.x; e < nbox; e += kThreads) {
+        const float val = box[e];
+        if (val == 0.0f) continue;
+        const int c = e / v3, r = e % v3;
+        deposit_at(J[c], d, sink.o[0] + r / v1, sink.o[1] + (r / sink.e[2]) % sink.e[1], sink.o[2] + r % sink.e[2],
+                   val);
+    }
 }
 
 }  // namespace
