@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define PIC_ABI_VERSION 2
+#define PIC_ABI_VERSION 3
 #define PIC_MAX_SPECIES 16
 
 /* Scalar run description: StaticParameters + DynamicParameters fields the hot
@@ -154,11 +154,18 @@ int pic_fused_push_esirkepov(const PicDesc* desc, const PicSpecies* sp, const Pi
  * pic_fused_push_esirkepov up to FP32 summation order.  E+E_ext and B+B_ext
  * around the tile are staged in LDS for the gather.  tile: 4 or 8 cells per
  * axis; rounds: 2 or 4 particle slots per thread per pass; min_blocks: 1, 3
- * or 4 resident blocks per CU requested from the register allocator. */
+ * or 4 resident blocks per CU requested from the register allocator.
+ * On an exact grid (every node origin + a*spacing an FP32 number, e.g. integer
+ * or dyadic spacings and origins) a thread takes `rounds` consecutive sorted
+ * slots and sums equal-cell deposits in registers; particles its fast loop does
+ * not cover are listed in workspace (1 + count*slots int32: workspace[0] is
+ * the list length) and finished by a second kernel in the same call.
+ * desc->variant = 1 selects the strided kernel on every grid (comparison). */
 int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
                                    const float* const E[3], const float* const B[3], const float* const E_ext[3],
                                    const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,
-                                   const int32_t* starts, int32_t rounds, int32_t min_blocks, void* stream);
+                                   const int32_t* starts, int32_t rounds, int32_t min_blocks, int32_t* workspace,
+                                   void* stream);
 
 /* ---- cell sort (particle_initialization.py:89-102, 140-160) ------------- */
 /* key = s*(N+1) + (ix*Ny+iy)*Nz+iz with ix = clip(floor((x+W/2)/dx),0,Nx-1);

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libpypic3d_hip.so")
 MAX_SPECIES = 16
 PUSHERS = {"boris": 0, "higuera_cary": 1}
 FUSED_VARIANTS = 6
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 c_int = ctypes.c_int
 c_i32 = ctypes.c_int32
@@ -76,7 +76,7 @@ SIGNATURES = {
     "pic_deposit_rho_tiled": (c_int, [DP, SP, PP, c_vp, c_i32, c_vp, c_vp]),
     "pic_fused_push_esirkepov": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, PP, P3, c_vp]),
     "pic_fused_push_esirkepov_tiled": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, PP, P3, c_i32, c_vp, c_i32, c_i32,
-                                               c_vp]),
+                                               c_vp, c_vp]),
     "pic_cell_keys": (c_int, [DP, SP, PP, c_vp, c_vp]),
     "pic_tile_count": (c_i64, [DP, c_i32]),
     "pic_tile_keys": (c_int, [DP, SP, PP, c_i32, c_vp, c_vp]),

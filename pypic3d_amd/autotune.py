@@ -47,14 +47,15 @@ def tune_fused(desc, sp, pp_in, E3, B3, Ee, Be, pp_out, J3, reps=3, variants=Non
     return best, times
 
 
-def tune_tiled(desc, sp, pp_in, E3, B3, Ee, Be, pp_out, J3, tile, starts, reps=3, rounds=(2, 4),
+def tune_tiled(desc, sp, pp_in, E3, B3, Ee, Be, pp_out, J3, tile, starts, workspace, reps=3, rounds=(2, 4),
                min_blocks=(1, 3, 4)):
     """Launch shape of the tile-ordered kernel: ((rounds, min_blocks), {"rounds/min_blocks": mean ms})."""
     s = N.stream_handle()
     times = {}
     for r in rounds:
         for mb in min_blocks:
-            args = (desc, sp, pp_in, N.p3(E3), N.p3(B3), Ee, Be, pp_out, N.p3(J3), tile, N.ptr(starts), r, mb, s)
+            args = (desc, sp, pp_in, N.p3(E3), N.p3(B3), Ee, Be, pp_out, N.p3(J3), tile, N.ptr(starts), r, mb,
+                    N.ptr(workspace), s)
             N.call("pic_fused_push_esirkepov_tiled", *args)
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record()

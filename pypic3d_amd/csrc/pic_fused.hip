@@ -1,5 +1,8 @@
 // Single-simulation launch of the fused particle phase (the block function
 // and its design notes live in pic_fused_block.h).
+#include <algorithm>
+#include <cmath>
+
 #include "pic_fused_block.h"
 
 namespace {
@@ -87,6 +90,415 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tiled_kernel(PicDe
     }
 }
 
+// Grouped variant of the tile-ordered launch.  Thread t of a chunk owns kP
+// consecutive sorted slots [base + kP t, base + kP t + kP), loaded and stored
+// as one kP-wide vector per plane when the group lies inside the block's
+// range and the planes are aligned.  Sorted slots of one cell are contiguous,
+// so a thread's particles mostly share a cell: their twelve regular-case
+// values are summed in registers and added into the LDS box when the cell
+// changes and once at the end of the group, which replaces the wave
+// segmented scan.  Face crossings go through the block's queue as in
+// fused_block.
+// Per-block LDS queue of face-crossing particles (slot offset from the chunk
+// base and the pushed velocity); a full queue sends further particles to the
+// rare list.
+constexpr int kCrossCap = 512;
+struct CrossQueue {
+    int q_idx[kCrossCap];
+    float q_v[3][kCrossCap];
+    int q_n;
+};
+
+// Add the twelve regular-case values of a particle anchored at cell
+// (a0, a1, a2) (RegularLayout<3> order) into the box, or into J when the
+// 2x2x2 node block around the cell leaves the box.
+template <int kB>
+__device__ inline void flush_cell(const PicDesc& d, const BoxSink& sink, int a0, int a1, int a2, const float acc[12]) {
+    const int bi = a0 - sink.o[0], bj = a1 - sink.o[1], bk = a2 - sink.o[2];
+    if ((unsigned)bi < (unsigned)(kB - 1) && (unsigned)bj < (unsigned)(kB - 1) && (unsigned)bk < (unsigned)(kB - 1)) {
+        float* p = sink.box + (bi * kB + bj) * kB + bk;
+#pragma unroll
+        for (int m = 0; m < 12; ++m) {
+            int dx, dy, dz;
+            RegularLayout<3>::delta(m, dx, dy, dz);
+            atomicAdd(p + (m >> 2) * kB * kB * kB + (dx * kB + dy) * kB + dz, acc[m]);
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < 12; ++m) {
+            int dx, dy, dz;
+            RegularLayout<3>::delta(m, dx, dy, dz);
+            deposit_at(sink.J[m >> 2], d, a0 + dx, a1 + dy, a2 + dz, acc[m]);
+        }
+    }
+}
+
+// Grid constants of an exact grid: on every axis both node lines have an FP32
+// origin and spacing and every node org + a*sp with a in [lo, hi] is an FP32
+// number (checked on the host).  Then the anchor of an FP32 position is found
+// in FP32 by bracketing it between exact node coordinates, and x - node is
+// the correctly rounded difference -- the value the FP64 evaluation rounds to
+// -- so no FP64 arithmetic is needed for the old position.
+struct ExactGrid {
+    float org_c[3], sp_c[3], isp_c[3];
+    float org_v[3], sp_v[3], isp_v[3];
+    float inv_d[3];
+    int lo[3], hi[3];
+    double tol[3];      // band around a face inside which a new FP64 position takes the exact path
+};
+
+__device__ inline bool exact_anchor(float x, float org, float sp, float isp, int lo, int hi, int& a, float& dl,
+                                    float& node) {
+    float f = floorf((x - org) * isp);
+    float nd = fmaf(f, sp, org);
+    const bool below = x < nd, above = x >= nd + sp;
+    f = below ? f - 1.0f : (above ? f + 1.0f : f);
+    nd = below ? nd - sp : (above ? nd + sp : nd);
+    const bool ok = x >= nd && x < nd + sp && f >= (float)lo && f <= (float)hi;
+    a = ok ? (int)f : 0;
+    dl = x - nd;
+    node = nd;
+    return ok;
+}
+
+// One particle of the tile kernel through the general route: global gather
+// with FP64 anchors, push, 3-slot (or 5-slot) Esirkepov deposit, position
+// update, boundaries and store.  Run by fused_tile_rare_kernel for the rare
+// particles the tile kernel's fast loop does not cover.
+template <class Sink>
+__device__ inline void tile_particle_general(const PicDesc& d, const SpeciesK& k, const PicParticles& in,
+                                             const FieldPtrs& fp, const PicParticles& out, const Sink& sink,
+                                             int64_t i) {
+    double xo[3];
+    load_pos(in, i, xo);
+    float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
+    CicAxis gc[3], gv[3];
+    int anc[3], anv[3];
+    float dc[3], dv[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float x = (float)xo[a];
+        cic_axis(x, (float)d.org_c[a], (float)(1.0 / d.sp_c[a]), d.org_c[a], d.sp_c[a], d.dims[a], gc[a], anc[a],
+                 dc[a]);
+        cic_axis(x, (float)d.org_v[a], (float)(1.0 / d.sp_v[a]), d.org_v[a], d.sp_v[a], d.dims[a], gv[a], anv[a],
+                 dv[a]);
+    }
+    float E[3], B[3];
+    E[0] = gather_cic(fp.E[0], fp.Ee[0], d, gv[0], gc[1], gc[2]);
+    E[1] = gather_cic(fp.E[1], fp.Ee[1], d, gc[0], gv[1], gc[2]);
+    E[2] = gather_cic(fp.E[2], fp.Ee[2], d, gc[0], gc[1], gv[2]);
+    B[0] = gather_cic(fp.B[0], fp.Be[0], d, gc[0], gv[1], gv[2]);
+    B[1] = gather_cic(fp.B[1], fp.Be[1], d, gv[0], gc[1], gv[2]);
+    B[2] = gather_cic(fp.B[2], fp.Be[2], d, gv[0], gv[1], gc[2]);
+    float nv[3] = {v[0], v[1], v[2]};
+    push_velocity(nv, E, B, k.h, d);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if ((k.uu >> a) & 1u) v[a] = nv[a];
+    double xn[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) xn[a] = ((k.ux >> a) & 1u) ? step_pos(xo[a], (double)v[a], d.dt) : xo[a];
+    CrossAxis cx, cy, cz;
+    bool ok = cross_axis(xo[0], xn[0], d.org_c[0], d.sp_c[0], 1.0 / d.d[0], (float)d.org_c[0],
+                         (float)(1.0 / d.sp_c[0]), cx);
+    ok = cross_axis(xo[1], xn[1], d.org_c[1], d.sp_c[1], 1.0 / d.d[1], (float)d.org_c[1], (float)(1.0 / d.sp_c[1]),
+                    cy) && ok;
+    ok = cross_axis(xo[2], xn[2], d.org_c[2], d.sp_c[2], 1.0 / d.d[2], (float)d.org_c[2], (float)(1.0 / d.sp_c[2]),
+                    cz) && ok;
+    if (ok) {
+        crossing_deposit(d, k.qw, cx, cy, cz, sink);
+    } else {
+        float* const J[3] = {sink.J[0], sink.J[1], sink.J[2]};
+        esirkepov_particle(d, k.qw, xo, xn, v, J);
+    }
+    bool act = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) apply_axis_boundary(d.pbc[a], d.wind[a], xn[a], v[a], act);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        out.x[a][i] = act ? (float)xn[a] : 0.0f;
+        out.u[a][i] = act ? v[a] : 0.0f;
+    }
+    out.active[i] = act ? 1 : 0;
+}
+
+// Tile-ordered launch on an exact grid.  Block b owns the sorted slot range
+// [starts[b], starts[b+1]) of one (species, tile) pair.  Thread t of a chunk
+// owns kP consecutive slots; sorted slots of one cell are contiguous, so a
+// thread's particles mostly share a cell and their twelve regular-case values
+// are summed in registers and added into the LDS current box when the cell
+// changes and at the end of the group (no cross-lane reduction).  Fields are
+// gathered from the LDS field box.  Particles that cross a face go to the
+// block's crossing queue (3-slot deposit into the box after the chunk).
+// Anything else -- a particle outside the tile's field box or the exact
+// anchor range, a step of a cell or more on some axis, a crossing that finds
+// the queue full -- is appended to the rare list (rare[0] = count, slots from
+// rare[1]) without being stored; fused_tile_rare_kernel then runs it whole.
+// Keeping those paths out of this kernel keeps its register count low.
+
+__device__ inline void rare_push(int32_t* rare, int64_t i) {
+    const int p = atomicAdd(rare, 1);
+    rare[1 + p] = (int32_t)i;
+}
+
+template <int kT, int kP, int kMinBlocks>
+__global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
+    PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, PicParticles out, float* Jx, float* Jy, float* Jz,
+    const int32_t* __restrict__ starts, TileGeom tg, ExactGrid eg, int32_t* __restrict__ rare) {
+    constexpr int kB = kT + 4;
+    constexpr int kB3 = kB * kB * kB;
+    constexpr int kFB = kT + 4;
+    constexpr int kFV = kFB * kFB * kFB;
+    constexpr int kChunk = kThreads * kP;
+    __shared__ CrossQueue sh;
+    __shared__ float box[3 * kB3];
+    __shared__ float fbox[6 * kFV];
+    const int b = blockIdx.x;
+    const int64_t begin = starts[b], end = starts[b + 1];
+    if (begin >= end) return;
+    const int t = b % (tg.tiles + 1);
+    const int s = b / (tg.tiles + 1);          // blocks are grouped per species
+    int o[3] = {0, 0, 0}, fo[3] = {0, 0, 0};
+    if (t < tg.tiles) {
+        const int tx = t / (tg.nt[1] * tg.nt[2]), ty = (t / tg.nt[2]) % tg.nt[1], tz = t % tg.nt[2];
+        const int c0[3] = {tx * kT, ty * kT, tz * kT};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            o[a] = c0[a] + d.g - 1;
+            fo[a] = c0[a] + d.g - 2;
+        }
+    }
+    for (int e = threadIdx.x; e < 3 * kB3; e += kThreads) box[e] = 0.0f;
+    for (int e = threadIdx.x; e < 6 * kFV; e += kThreads) {
+        const int c = e / kFV, r = e % kFV;
+        const int i = pmod(fo[0] + r / (kFB * kFB), d.dims[0]), j = pmod(fo[1] + (r / kFB) % kFB, d.dims[1]);
+        const int k = pmod(fo[2] + r % kFB, d.dims[2]);
+        const int64_t id = fidx(d, i, j, k);
+        const float* F = c < 3 ? fp.E[c] : fp.B[c - 3];
+        const float* Fe = c < 3 ? fp.Ee[c] : fp.Be[c - 3];
+        float v = F[id];
+        if (Fe) v += Fe[id];
+        fbox[e] = v;
+    }
+    const BoxSink sink{{Jx, Jy, Jz}, box, {o[0], o[1], o[2]}, kB};
+    const FieldBox fb{fbox, {fo[0], fo[1], fo[2]}, kFB};
+    const SpeciesK k = species_constants(d, sp, s);
+    const float fdt = (float)d.dt;
+
+    for (int64_t base = begin & ~(int64_t)(kP - 1); base < end; base += kChunk) {
+        if (threadIdx.x == 0) sh.q_n = 0;
+        __syncthreads();    // also orders the box fills before the first deposits
+        const int64_t s0 = base + (int64_t)kP * threadIdx.x;
+        float acc[12];
+#pragma unroll
+        for (int m = 0; m < 12; ++m) acc[m] = 0.0f;
+        int h0 = 0, h1 = 0, h2 = 0;
+        bool have = false;
+        // one particle per iteration; the loop stays rolled to bound register pressure
+#pragma unroll 1
+        for (int j = 0; j < kP; ++j) {
+            const int64_t i = s0 + j;
+            if (i < begin || i >= end) continue;
+            const bool act0 = in.active[i] != 0;
+            const float xf[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
+            float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
+            if (!act0) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    out.x[a][i] = 0.0f;
+                    out.u[a][i] = 0.0f;
+                }
+                out.active[i] = 0;
+                continue;
+            }
+            // ---- anchors on both node lines (grid_and_stencil.py:92-166), exact in FP32
+            int anc[3], anv[3];
+            float dc[3], dv[3], nodec[3], nodev;
+            bool fast = true;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                fast = exact_anchor(xf[a], eg.org_c[a], eg.sp_c[a], eg.isp_c[a], eg.lo[a], eg.hi[a], anc[a], dc[a],
+                                    nodec[a]) && fast;
+                fast = exact_anchor(xf[a], eg.org_v[a], eg.sp_v[a], eg.isp_v[a], eg.lo[a], eg.hi[a], anv[a], dv[a],
+                                    nodev) && fast;
+                fast = fast && (unsigned)(anc[a] - fo[a]) < (unsigned)(kFB - 1) &&
+                       (unsigned)(anv[a] - fo[a]) < (unsigned)(kFB - 1);
+            }
+            if (!fast) {
+                rare_push(rare, i);
+                continue;
+            }
+            // ---- gather (boris.py:302-433) from the tile's field box
+            CicAxis gc[3], gv[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float rc = dc[a] * eg.isp_c[a], rv = dv[a] * eg.isp_v[a];
+                gc[a].p0 = gc[a].p1 = gv[a].p0 = gv[a].p1 = 0;
+                gc[a].w0 = 1.0f - rc;
+                gc[a].w1 = rc;
+                gv[a].w0 = 1.0f - rv;
+                gv[a].w1 = rv;
+            }
+            float E[3], Bf[3];
+            E[0] = gather_box(fb, 0, anv[0], anc[1], anc[2], gv[0], gc[1], gc[2]);
+            E[1] = gather_box(fb, 1, anc[0], anv[1], anc[2], gc[0], gv[1], gc[2]);
+            E[2] = gather_box(fb, 2, anc[0], anc[1], anv[2], gc[0], gc[1], gv[2]);
+            Bf[0] = gather_box(fb, 3, anc[0], anv[1], anv[2], gc[0], gv[1], gv[2]);
+            Bf[1] = gather_box(fb, 4, anv[0], anc[1], anv[2], gv[0], gc[1], gv[2]);
+            Bf[2] = gather_box(fb, 5, anv[0], anv[1], anc[2], gv[0], gv[1], gc[2]);
+            // ---- push (boris.py:14-124), masked by update_u (particle_push.py:134-142)
+            float nv[3] = {v[0], v[1], v[2]};
+            push_velocity(nv, E, Bf, k.h, d);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                if ((k.uu >> a) & 1u) v[a] = nv[a];
+            // ---- new position (Esirkepov.py:125-127, FP64) and regular-case test: the
+            // new position stays in the old cell, away from its faces by more than the
+            // FP64 rounding of floor((x - origin)/spacing)
+            double xn[3];
+            bool regular = true, short_step = true;
+            float S0[3][2], D[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const bool ux = (k.ux >> a) & 1u;
+                xn[a] = ux ? step_pos((double)xf[a], (double)v[a], d.dt) : (double)xf[a];
+                const double lo = (double)nodec[a] + eg.tol[a], hi = (double)(nodec[a] + eg.sp_c[a]) - eg.tol[a];
+                regular = regular && (!ux || (xn[a] >= lo && xn[a] < hi)) && anc[a] >= 0 && anc[a] + 1 < d.dims[a];
+                const float dvt = ux ? v[a] * fdt : 0.0f;
+                short_step = short_step && fabsf(dvt) < 0.99f * eg.sp_c[a];   // anchor moves by one node at most
+                const float r0 = dc[a] * eg.inv_d[a];
+                S0[a][0] = 1.0f - r0;
+                S0[a][1] = r0;
+                D[a] = dvt * eg.inv_d[a];
+            }
+            if (!regular) {
+                const int slot = short_step ? atomicAdd(&sh.q_n, 1) : kCrossCap;
+                if (slot >= kCrossCap) {
+                    rare_push(rare, i);
+                    continue;
+                }
+                sh.q_idx[slot] = kP * threadIdx.x + j;
+                sh.q_v[0][slot] = v[0];
+                sh.q_v[1][slot] = v[1];
+                sh.q_v[2][slot] = v[2];
+            }
+            if (regular) {
+                // difference-form Esirkepov weights of a particle that stays in its
+                // cell (Esirkepov.py:158-214): W = D_c (S0 S0 + (S0 D + D S0)/2 + D D/3)
+                // over the other two axes, J_c = -(q/(d_o1 d_o2 dt)) W at slot a only
+                float val[12];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int o1 = c == 0 ? 1 : 0, o2 = c == 2 ? 1 : 2;
+                    const float pre = k.pre[c] * (-D[c]);
+#pragma unroll
+                    for (int jj = 0; jj < 2; ++jj) {
+                        const float dj = jj ? D[o1] : -D[o1];
+#pragma unroll
+                        for (int kk = 0; kk < 2; ++kk) {
+                            const float dk = kk ? D[o2] : -D[o2];
+                            const float y = S0[o1][jj] * S0[o2][kk] + 0.5f * (S0[o1][jj] * dk + dj * S0[o2][kk]) +
+                                            (1.0f / 3.0f) * dj * dk;
+                            val[c * 4 + jj * 2 + kk] = pre * y;
+                        }
+                    }
+                }
+                const bool same = have && anc[0] == h0 && anc[1] == h1 && anc[2] == h2;
+                if (have && !same) flush_cell<kB>(d, sink, h0, h1, h2, acc);
+#pragma unroll
+                for (int m = 0; m < 12; ++m) acc[m] = same ? acc[m] + val[m] : val[m];
+                h0 = anc[0];
+                h1 = anc[1];
+                h2 = anc[2];
+                have = true;
+            }
+            // ---- position update + particle boundaries (particle_tile_communication.py)
+            bool act = true;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) apply_axis_boundary(d.pbc[a], d.wind[a], xn[a], v[a], act);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                out.x[a][i] = act ? (float)xn[a] : 0.0f;
+                out.u[a][i] = act ? v[a] : 0.0f;
+            }
+            out.active[i] = act ? 1 : 0;
+        }
+        if (have) flush_cell<kB>(d, sink, h0, h1, h2, acc);
+        __syncthreads();
+        // ---- queued face crossings (3-slot frame, Esirkepov.py:105-331)
+        const int nq = min(sh.q_n, kCrossCap);
+        for (int q = threadIdx.x; q < nq; q += kThreads) {
+            const int64_t i = base + sh.q_idx[q];
+            const float v[3] = {sh.q_v[0][q], sh.q_v[1][q], sh.q_v[2][q]};
+            double xo[3], xn[3];
+            load_pos(in, i, xo);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) xn[a] = ((k.ux >> a) & 1u) ? step_pos(xo[a], (double)v[a], d.dt) : xo[a];
+            CrossAxis cx, cy, cz;
+            bool ok = cross_axis(xo[0], xn[0], d.org_c[0], d.sp_c[0], 1.0 / d.d[0], eg.org_c[0], eg.isp_c[0], cx);
+            ok = cross_axis(xo[1], xn[1], d.org_c[1], d.sp_c[1], 1.0 / d.d[1], eg.org_c[1], eg.isp_c[1], cy) && ok;
+            ok = cross_axis(xo[2], xn[2], d.org_c[2], d.sp_c[2], 1.0 / d.d[2], eg.org_c[2], eg.isp_c[2], cz) && ok;
+            if (ok) crossing_deposit(d, k.qw, cx, cy, cz, sink);
+            else rare_push(rare, i);     // not reached after the step-length test; the rare path redoes it whole
+        }
+        __syncthreads();
+    }
+    float* const J[3] = {Jx, Jy, Jz};
+    for (int e = threadIdx.x; e < 3 * kB3; e += kThreads) {
+        const float v = box[e];
+        if (v == 0.0f) continue;
+        const int c = e / kB3, r = e % kB3;
+        deposit_at(J[c], d, o[0] + r / (kB * kB), o[1] + (r / kB) % kB, o[2] + r % kB, v);
+    }
+}
+
+// The rare list of fused_tile_exact_kernel: each listed particle runs the
+// general route with its deposit added straight into J.
+__global__ __launch_bounds__(kThreads) void fused_tile_rare_kernel(PicDesc d, PicSpecies sp, PicParticles in,
+                                                                   FieldPtrs fp, PicParticles out, float* Jx,
+                                                                   float* Jy, float* Jz,
+                                                                   const int32_t* __restrict__ rare) {
+    const int n = rare[0];
+    const GlobalSink sink{{Jx, Jy, Jz}};
+    for (int q = blockIdx.x * kThreads + threadIdx.x; q < n; q += gridDim.x * kThreads) {
+        const int64_t i = rare[1 + q];
+        const SpeciesK k = species_constants(d, sp, species_of(i, sp.slots, sp.count));
+        tile_particle_general(d, k, in, fp, out, sink, i);
+    }
+}
+
+__global__ void zero_count_kernel(int32_t* c) { *c = 0; }
+
+// Host check for ExactGrid (see its comment); false when any axis fails.
+bool make_exact_grid(const PicDesc& d, ExactGrid& eg) {
+    for (int a = 0; a < 3; ++a) {
+        const double org[2] = {d.org_c[a], d.org_v[a]}, spc[2] = {d.sp_c[a], d.sp_v[a]};
+        const int lo = -d.dims[a] - 4, hi = 2 * d.dims[a] + 4;
+        for (int gl = 0; gl < 2; ++gl) {
+            const float o32 = (float)org[gl], s32 = (float)spc[gl];
+            if ((double)o32 != org[gl] || (double)s32 != spc[gl] || !(spc[gl] > 0.0)) return false;
+            for (int n = lo; n <= hi + 1; ++n) {
+                const double node = (double)n * spc[gl] + org[gl];          // exact: 24-bit operands
+                const float n32 = std::fma((float)n, s32, o32);
+                if ((double)n32 != node) return false;
+            }
+        }
+        eg.org_c[a] = (float)d.org_c[a];
+        eg.sp_c[a] = (float)d.sp_c[a];
+        eg.isp_c[a] = (float)(1.0 / d.sp_c[a]);
+        eg.org_v[a] = (float)d.org_v[a];
+        eg.sp_v[a] = (float)d.sp_v[a];
+        eg.isp_v[a] = (float)(1.0 / d.sp_v[a]);
+        eg.inv_d[a] = (float)(1.0 / d.d[a]);
+        eg.lo[a] = lo;
+        eg.hi[a] = hi;
+        // floor((x - org)/sp) in FP64 can differ from the exact bracket only within a
+        // few FP64 ulps of the quotient (|quotient| < 2^31): a band of 2^-20 sp is ample
+        eg.tol[a] = std::ldexp(d.sp_c[a], -20);
+    }
+    return true;
+}
+
 // Slot-ordered launch for 1-D and 2-D first-order runs (and the general
 // path): each block accumulates J in an LDS box sized to its particles.
 template <int kRounds>
@@ -144,7 +556,8 @@ int pic_fused_push_esirkepov(const PicDesc* desc, const PicSpecies* sp, const Pi
 int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
                                    const float* const E[3], const float* const B[3], const float* const E_ext[3],
                                    const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,
-                                   const int32_t* starts, int32_t rounds, int32_t min_blocks, void* stream) {
+                                   const int32_t* starts, int32_t rounds, int32_t min_blocks, int32_t* workspace,
+                                   void* stream) {
     PIC_REQUIRE(desc_ok(desc) && particles_ok(sp, in) && particles_ok(sp, out) && E && B && J && starts);
     PIC_REQUIRE(fused_kind(*desc) == 3 && (tile == 4 || tile == 8) && (rounds == 2 || rounds == 4));
     PIC_REQUIRE(min_blocks == 1 || min_blocks == 3 || min_blocks == 4);
@@ -172,7 +585,32 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
         else if (min_blocks == 4) PIC_TILED_LAUNCH(R, T, 4); \
         else PIC_TILED_LAUNCH(R, T, 1);                 \
     } while (0)
-    if (tile == 8) {
+    ExactGrid eg;
+    if (desc->variant != 1 && make_exact_grid(*desc, eg)) {
+        PIC_REQUIRE(workspace);
+        hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, workspace);
+#define PIC_EXACT_LAUNCH(T, P, MB)                                                                                  \
+        hipLaunchKernelGGL((fused_tile_exact_kernel<T, P, MB>), dim3((unsigned)blocks), dim3(kThreads), 0, st,      \
+                           *desc, *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace)
+#define PIC_EXACT_MB(T, P)                                  \
+        do {                                                \
+            if (min_blocks == 3) PIC_EXACT_LAUNCH(T, P, 3); \
+            else if (min_blocks == 4) PIC_EXACT_LAUNCH(T, P, 4); \
+            else PIC_EXACT_LAUNCH(T, P, 1);                 \
+        } while (0)
+        if (tile == 8) {
+            if (rounds == 2) PIC_EXACT_MB(8, 2);
+            else PIC_EXACT_MB(8, 4);
+        } else {
+            if (rounds == 2) PIC_EXACT_MB(4, 2);
+            else PIC_EXACT_MB(4, 4);
+        }
+#undef PIC_EXACT_MB
+#undef PIC_EXACT_LAUNCH
+        const int64_t rare_grid = std::min<int64_t>(std::max<int64_t>(grid_1d(n, kThreads), 1), 1024);
+        hipLaunchKernelGGL(fused_tile_rare_kernel, dim3((unsigned)rare_grid), dim3(kThreads), 0, st, *desc, *sp, *in,
+                           fp, *out, J[0], J[1], J[2], workspace);
+    } else if (tile == 8) {
         if (rounds == 2) PIC_TILED_MB(2, 8);
         else PIC_TILED_MB(4, 8);
     } else {

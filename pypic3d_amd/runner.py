@@ -73,6 +73,7 @@ class ElectrodynamicRunner:
         self.keys = None
         self.sorts_done = 0
         self.tile, self.rounds, self.min_blocks, self.starts = None, int(rounds), int(min_blocks), None
+        self.tile_ws = None
         if tile is not None:
             self.set_tile(tile)
 
@@ -112,7 +113,7 @@ class ElectrodynamicRunner:
             if self.tile:
                 N.call("pic_fused_push_esirkepov_tiled", d, self.sp, self._pp(src), N.p3(E), N.p3(self.B), self.Ee,
                        self.Be, self._pp(dst), N.p3(self.J), self.tile, N.ptr(self.starts), self.rounds,
-                       self.min_blocks, s)
+                       self.min_blocks, N.ptr(self.tile_ws), s)
             else:
                 N.call("pic_fused_push_esirkepov", d, self.sp, self._pp(src), N.p3(E), N.p3(self.B), self.Ee,
                        self.Be, self._pp(dst), N.p3(self.J), s)
@@ -162,7 +163,7 @@ class ElectrodynamicRunner:
         E = self.E[self.ecur]
         if self.tile:
             best, times = tune_tiled(self.desc, self.sp, self._pp(src), E, self.B, self.Ee, self.Be, self._pp(dst),
-                                     self.J, self.tile, self.starts, reps=reps)
+                                     self.J, self.tile, self.starts, self.tile_ws, reps=reps)
             self.rounds, self.min_blocks = best
             return best, times
         best, times = tune_fused(self.desc, self.sp, self._pp(src), E, self.B, self.Ee, self.Be, self._pp(dst),
@@ -184,7 +185,7 @@ class ElectrodynamicRunner:
         """Switch to the tile-ordered kernel with `tile`^3-cell tiles (None:
         the slot-ordered kernel) and sort the particles accordingly."""
         if tile is None:
-            self.tile, self.starts = None, None
+            self.tile, self.starts, self.tile_ws = None, None, None
             return
         d = self.desc
         if self.esirkepov and not (d.shape_factor == 1 and all(d.n[a] > 1 for a in range(3))):
@@ -194,6 +195,8 @@ class ElectrodynamicRunner:
         self.tile = int(tile)
         tiles = int(N.lib().pic_tile_count(d, self.tile))
         self.starts = torch.empty(self.S * (tiles + 1) + 1, dtype=torch.int32, device=self.device)
+        # the tiled kernel's list of particles for its general route (1 + slots int32)
+        self.tile_ws = torch.zeros(1 + self.S * self.P, dtype=torch.int32, device=self.device)
         self.sort()
 
     def sort(self):

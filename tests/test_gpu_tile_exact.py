@@ -1,0 +1,111 @@
+"""Tile-ordered fused kernel on exact grids (FP32 anchors by bracketing exact
+node coordinates, register-summed deposits, rare-list second kernel) against
+the strided tile kernel on the same launch, and whole runs against the FP64
+oracle, including particles that leave their tile and steps longer than a
+cell (both handled by the rare-list kernel)."""
+
+import numpy as np
+import pytest
+import torch
+
+from pypic3d_amd import _native as N
+from pypic3d_amd import workloads
+from pypic3d_amd.evolve import time_loop_electrodynamic
+from pypic3d_amd.runner import ElectrodynamicRunner
+from tests.gpu_helpers import gpu_field, host, host_particles, oracle_params, rel_err
+from tests.helpers import random_field
+
+pytestmark = pytest.mark.gpu
+
+
+def _runner(N_, wind, specs, tile, rounds, min_blocks, dt=None, ext=True, seed=3):
+    static, dyn = workloads.make_parameters(N_, wind=wind, cfl=0.5, dt=dt)
+    parts, cfg = workloads.plasma(static, dyn, specs, seed=seed)
+    p = oracle_params(static, dyn)
+    fields = workloads.empty_fields(static, dyn)
+    E0 = tuple(gpu_field(random_field(p, 40 + c, scale=0.05)) for c in range(3))
+    B0 = tuple(gpu_field(random_field(p, 43 + c, scale=0.05)) for c in range(3))
+    if ext:
+        Ee = tuple(gpu_field(random_field(p, 46 + c, scale=0.02)) for c in range(3))
+        Be = tuple(gpu_field(random_field(p, 49 + c, scale=0.02)) for c in range(3))
+        fields = (E0, B0) + fields[2:5] + ((Ee, Be),) + fields[6:]
+    else:
+        fields = (E0, B0) + fields[2:]
+    r = ElectrodynamicRunner(parts, cfg, fields, static, dyn, sort_every=0, tile=tile, rounds=rounds,
+                             min_blocks=min_blocks)
+    return r, parts, cfg, fields, static, dyn
+
+
+def _launch(r, variant):
+    src, dst = r.cur, 1 - r.cur
+    r.desc.variant = variant
+    for c in r.J:
+        c.zero_()
+    r.tile_ws[0] = -7
+    N.call("pic_fused_push_esirkepov_tiled", r.desc, r.sp, r._pp(src), N.p3(r.E[r.ecur]), N.p3(r.B), r.Ee, r.Be,
+           r._pp(dst), N.p3(r.J), r.tile, N.ptr(r.starts), r.rounds, r.min_blocks, N.ptr(r.tile_ws),
+           N.stream_handle())
+    torch.cuda.synchronize()
+    r.desc.variant = 0
+    return (torch.stack(r.J).clone(), [t.clone() for t in r.pos[dst]], [t.clone() for t in r.vel[dst]],
+            r.act[dst].clone(), int(r.tile_ws[0]))
+
+
+@pytest.mark.parametrize("tile,rounds,min_blocks", [(4, 4, 4), (4, 2, 1), (8, 4, 3)])
+def test_exact_tile_kernel_agrees_with_strided_tile_kernel(tile, rounds, min_blocks):
+    """Same sorted input, one launch each: identical particles, J equal up to
+    FP32 summation order; some particles have drifted out of their tile or
+    wrapped through the periodic faces and take the rare list."""
+    specs = [workloads.SpeciesSpec(24, weight=0.01, vth=(0.2,) * 3),
+             workloads.SpeciesSpec(7, charge=1.0, mass=20.0, weight=0.03, vth=(0.05,) * 3)]
+    r, *_ = _runner((20, 16, 12), (10.0, 8.0, 6.0), specs, tile, rounds, min_blocks)
+    for _ in range(6):          # drift away from the sorted order (no re-sort)
+        r.step()
+    J1, x1, u1, a1, nr1 = _launch(r, 1)
+    J0, x0, u0, a0, nr0 = _launch(r, 0)
+    assert nr1 == -7                                  # strided kernel leaves the workspace alone
+    assert 0 < nr0 < a0.numel() // 10
+    for p, q in zip(x0 + u0, x1 + u1):
+        assert torch.equal(p, q)
+    assert torch.equal(a0, a1)
+    assert float((J0 - J1).norm() / J1.norm()) <= 1e-6
+
+
+def test_non_exact_grid_uses_the_strided_kernel():
+    specs = [workloads.SpeciesSpec(8, weight=0.01, vth=(0.1,) * 3)]
+    r, *_ = _runner((12, 10, 8), (1.3, 1.1, 0.7), specs, 4, 4, 1, ext=False)
+    r.step()
+    J0, x0, u0, a0, nr0 = _launch(r, 0)
+    J1, x1, u1, a1, nr1 = _launch(r, 1)
+    assert nr0 == -7 and nr1 == -7
+    for p, q in zip(x0 + u0, x1 + u1):
+        assert torch.equal(p, q)
+    assert float((J0 - J1).norm() / J1.norm()) <= 1e-6
+
+
+@pytest.mark.parametrize("dt,steps", [(None, 6), (3.0, 2)])
+def test_exact_tile_runner_matches_functional_step(dt, steps):
+    """Tile runner without re-sorting against time_loop_electrodynamic (FP64
+    oracle semantics); dt = 3 on a unit grid moves the fast particles more
+    than a cell per step, through the rare-list kernel's general route."""
+    specs = [workloads.SpeciesSpec(12, weight=0.01, vth=(0.15,) * 3),
+             workloads.SpeciesSpec(5, charge=1.0, mass=20.0, weight=0.03, vth=(0.05,) * 3)]
+    r, parts, cfg, fields, static, dyn = _runner((16, 12, 10), (16.0, 12.0, 10.0), specs, 4, 4, 1, dt=dt)
+    ref_parts, ref_fields = parts, fields
+    for _ in range(steps):
+        r.step()
+        ref_parts, ref_fields = time_loop_electrodynamic(ref_parts, cfg, ref_fields, static, dyn)
+    if dt is not None:
+        assert int(r.tile_ws[0]) > 0
+    r_parts, r_fields = r.state()
+    xa, ua, aa = host_particles(r_parts)
+    xb, ub, ab = host_particles(ref_parts)
+    for s_ in range(2):
+        oa = np.lexsort((xa[s_, :, 2], xa[s_, :, 1], xa[s_, :, 0]))
+        ob = np.lexsort((xb[s_, :, 2], xb[s_, :, 1], xb[s_, :, 0]))
+        np.testing.assert_array_equal(aa[s_, oa], ab[s_, ob])
+        assert np.max(np.abs(xa[s_, oa] - xb[s_, ob])) <= 1e-5
+        assert rel_err(ua[s_, oa], ub[s_, ob]) <= 1e-5
+    for k in (0, 1, 2):
+        for c in range(3):
+            assert rel_err(host(r_fields[k][c]), host(ref_fields[k][c])) <= 1e-5, (k, c)
