@@ -242,12 +242,14 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--ppc", type=int, default=32)
     ap.add_argument("--sort-every", type=int, default=10)
-    ap.add_argument("--tile", type=int, default=8, choices=(0, 4, 8),
+    ap.add_argument("--tile", type=int, default=4, choices=(0, 4, 8),
                     help="cells per tile edge of the tile-ordered fused kernel (0: slot-ordered kernel)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--variant", type=int, default=None,
-                    help="use this fused-kernel launch shape instead of autotuning")
+                    help="use this fused-kernel launch shape instead of autotuning (tile mode: slots per thread)")
+    ap.add_argument("--min-blocks", type=int, default=None, choices=(1, 3, 4),
+                    help="tile mode with --variant: resident-block target of the fused kernel")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble comparison")
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
@@ -272,6 +274,7 @@ def main():
         sys.exit(2)
 
     import torch
+    from pypic3d_amd import _native as N
     from pypic3d_amd import workloads
     from pypic3d_amd.runner import ElectrodynamicRunner
 
@@ -292,7 +295,9 @@ def main():
     torch.cuda.synchronize()
     if args.variant is not None:
         if runner.tile:
-            runner.rounds = variant = args.variant
+            runner.rounds = args.variant
+            runner.min_blocks = args.min_blocks or runner.min_blocks
+            variant = (runner.rounds, runner.min_blocks)
         else:
             runner.desc.variant = variant = args.variant
         tune_ms = {}
@@ -315,6 +320,11 @@ def main():
     sorts = runner.sorts_done - sorts0
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s; phases {phases}", file=sys.stderr, flush=True)
 
+    if runner.tile:
+        exact = bool(N.lib().pic_tile_exact_grid(runner.desc)) and runner.desc.variant != 1
+        kernel_name = "fused_tile_exact_kernel" if exact else "fused_tiled_kernel"
+    else:
+        kernel_name = "fused_push_esirkepov_v2"
     ms_per_step = 1e3 * elapsed / args.steps
     value = n_particles * args.steps / elapsed
     kern_ms = phases.get("particles")
@@ -343,7 +353,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "fused_tiled_kernel" if runner.tile else "fused_push_esirkepov_v2",
+            "kernel": kernel_name,
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": None, "kernel_ms": kern_ms, "algorithmic_bytes": kern_bytes,

@@ -160,7 +160,9 @@ int pic_fused_push_esirkepov(const PicDesc* desc, const PicSpecies* sp, const Pi
  * slots and sums equal-cell deposits in registers; particles its fast loop does
  * not cover are listed in workspace (1 + count*slots int32: workspace[0] is
  * the list length) and finished by a second kernel in the same call.
- * desc->variant = 1 selects the strided kernel on every grid (comparison). */
+ * desc->variant = 1 selects the strided kernel on every grid (comparison).
+ * pic_tile_exact_grid returns 1 when desc is such an exact grid. */
+int pic_tile_exact_grid(const PicDesc* desc);
 int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
                                    const float* const E[3], const float* const B[3], const float* const E_ext[3],
                                    const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,

@@ -553,6 +553,11 @@ int pic_fused_push_esirkepov(const PicDesc* desc, const PicSpecies* sp, const Pi
     return 0;
 }
 
+int pic_tile_exact_grid(const PicDesc* desc) {
+    ExactGrid eg;
+    return desc && desc_ok(desc) && make_exact_grid(*desc, eg) ? 1 : 0;
+}
+
 int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
                                    const float* const E[3], const float* const B[3], const float* const E_ext[3],
                                    const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,
