@@ -133,32 +133,60 @@ __device__ inline void flush_cell(const PicDesc& d, const BoxSink& sink, int a0,
     }
 }
 
-// Grid constants of an exact grid: on every axis both node lines have an FP32
-// origin and spacing and every node org + a*sp with a in [lo, hi] is an FP32
-// number (checked on the host).  Then the anchor of an FP32 position is found
-// in FP32 by bracketing it between exact node coordinates, and x - node is
-// the correctly rounded difference -- the value the FP64 evaluation rounds to
-// -- so no FP64 arithmetic is needed for the old position.
+// Grid constants of an exact grid: on every axis the center line has an FP32
+// origin and spacing, the vertex line is the center line shifted by half a
+// spacing, and every node org + a*sp with a in [lo, hi + 1] (both lines) is an
+// FP32 number (checked on the host).  For a position inside [xlo, xhi) the
+// anchor of an FP32 position is then found in FP32 by bracketing it between
+// exact node coordinates, and x - node is the correctly rounded difference --
+// the value the FP64 evaluation rounds to -- so no FP64 arithmetic is needed
+// for the old position.
 struct ExactGrid {
-    float org_c[3], sp_c[3], isp_c[3];
-    float org_v[3], sp_v[3], isp_v[3];
+    float org_c[3], sp_c[3], isp_c[3], half_sp[3];
     float inv_d[3];
-    int lo[3], hi[3];
-    double tol[3];      // band around a face inside which a new FP64 position takes the exact path
+    float xlo[3], xhi[3];
+    double tol[3];      // band around a face inside which a new FP64 position takes the queue
 };
 
-__device__ inline bool exact_anchor(float x, float org, float sp, float isp, int lo, int hi, int& a, float& dl,
-                                    float& node) {
+// Center-line anchor and offset of x (xlo <= x < xhi): the FP32 estimate is
+// within one node of the anchor, one bracketing step against the exact nodes
+// corrects it.  The vertex line follows from the half-cell test.
+__device__ inline void exact_anchors(float x, float org, float sp, float isp, float half_sp, int& ac, float& dc,
+                                     float& nodec, int& av, float& dv) {
     float f = floorf((x - org) * isp);
     float nd = fmaf(f, sp, org);
     const bool below = x < nd, above = x >= nd + sp;
     f = below ? f - 1.0f : (above ? f + 1.0f : f);
     nd = below ? nd - sp : (above ? nd + sp : nd);
-    const bool ok = x >= nd && x < nd + sp && f >= (float)lo && f <= (float)hi;
-    a = ok ? (int)f : 0;
-    dl = x - nd;
-    node = nd;
-    return ok;
+    ac = (int)f;
+    dc = x - nd;
+    nodec = nd;
+    const float mid = nd + half_sp;          // vertex node of index ac
+    const bool upper = x >= mid;
+    av = upper ? ac : ac - 1;
+    dv = upper ? x - mid : x - (mid - sp);
+}
+
+// Periodic wrap of wrap_periodic_position (grid_and_stencil.py:14-35) with
+// the one-period cases done without fmod (fmod(s, W) = s - W exactly for
+// W <= s < 2W, and = s for -W < s < 0); anything else takes the general form.
+__device__ inline void periodic_wrap(double wind, double& x) {
+    const double half = 0.5 * wind;
+    if (x >= half || x < -half) {
+        const double s = x + half;
+        double m;
+        if (s >= wind && s < 2.0 * wind) {
+            m = s - wind;
+        } else if (s < 0.0 && s > -wind) {
+            m = s + wind;
+        } else {
+            m = fmod(s, wind);
+            if (m != 0.0 && ((m < 0.0) != (wind < 0.0))) m += wind;
+        }
+        double w = m - half;
+        if (w == -half && x >= half) w = half;
+        x = w;
+    }
 }
 
 // One particle of the tile kernel through the general route: global gather
@@ -241,6 +269,48 @@ __device__ inline void rare_push(int32_t* rare, int64_t i) {
     rare[1 + p] = (int32_t)i;
 }
 
+// Face-crossing deposit (the 54 terms of crossing_deposit) straight into the
+// LDS box when the particle's 3x3x3 frame lies inside it: one base address,
+// the term offsets are compile-time constants, no per-term bounds checks.
+// pre[c] = -(q w / (d_p d_q dt)) as in SpeciesK.
+template <int kC, int kB>
+__device__ inline void crossing_component_box(const float pre, const CrossAxis& A, const CrossAxis& P,
+                                              const CrossAxis& Q, float* p) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const float ci = i == 0 ? A.C0 : A.C1;
+        if (ci == 0.0f) continue;
+        const float w = pre * ci;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float y = P.S0[j] * Q.S0[k] + 0.5f * (P.S0[j] * Q.D[k] + P.D[j] * Q.S0[k]) +
+                                (1.0f / 3.0f) * P.D[j] * Q.D[k];
+                const int dx = kC == 0 ? i : j;
+                const int dy = kC == 0 ? j : (kC == 1 ? i : k);
+                const int dz = kC == 2 ? i : k;
+                atomicAdd(p + kC * kB * kB * kB + (dx * kB + dy) * kB + dz, w * y);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);   // bound the live terms to one row of nine
+    }
+}
+
+// Returns false (nothing deposited) when the frame leaves the box.
+template <int kB>
+__device__ inline bool crossing_deposit_box(const SpeciesK& k, const CrossAxis& cx, const CrossAxis& cy,
+                                            const CrossAxis& cz, const BoxSink& sink) {
+    const int bi = cx.b - sink.o[0], bj = cy.b - sink.o[1], bk = cz.b - sink.o[2];
+    if ((unsigned)bi > (unsigned)(kB - 3) || (unsigned)bj > (unsigned)(kB - 3) || (unsigned)bk > (unsigned)(kB - 3))
+        return false;
+    float* p = sink.box + (bi * kB + bj) * kB + bk;
+    crossing_component_box<0, kB>(k.pre[0], cx, cy, cz, p);
+    crossing_component_box<1, kB>(k.pre[1], cy, cx, cz, p);
+    crossing_component_box<2, kB>(k.pre[2], cz, cx, cy, p);
+    return true;
+}
+
 template <int kT, int kP, int kMinBlocks>
 __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
     PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, PicParticles out, float* Jx, float* Jy, float* Jz,
@@ -313,14 +383,13 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
             }
             // ---- anchors on both node lines (grid_and_stencil.py:92-166), exact in FP32
             int anc[3], anv[3];
-            float dc[3], dv[3], nodec[3], nodev;
+            float dc[3], dv[3], nodec[3];
             bool fast = true;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                fast = exact_anchor(xf[a], eg.org_c[a], eg.sp_c[a], eg.isp_c[a], eg.lo[a], eg.hi[a], anc[a], dc[a],
-                                    nodec[a]) && fast;
-                fast = exact_anchor(xf[a], eg.org_v[a], eg.sp_v[a], eg.isp_v[a], eg.lo[a], eg.hi[a], anv[a], dv[a],
-                                    nodev) && fast;
+                fast = fast && xf[a] >= eg.xlo[a] && xf[a] < eg.xhi[a];
+                exact_anchors(xf[a], eg.org_c[a], eg.sp_c[a], eg.isp_c[a], eg.half_sp[a], anc[a], dc[a], nodec[a],
+                              anv[a], dv[a]);
                 fast = fast && (unsigned)(anc[a] - fo[a]) < (unsigned)(kFB - 1) &&
                        (unsigned)(anv[a] - fo[a]) < (unsigned)(kFB - 1);
             }
@@ -332,7 +401,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
             CicAxis gc[3], gv[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                const float rc = dc[a] * eg.isp_c[a], rv = dv[a] * eg.isp_v[a];
+                const float rc = dc[a] * eg.isp_c[a], rv = dv[a] * eg.isp_c[a];
                 gc[a].p0 = gc[a].p1 = gv[a].p0 = gv[a].p1 = 0;
                 gc[a].w0 = 1.0f - rc;
                 gc[a].w1 = rc;
@@ -384,24 +453,26 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
             }
             if (regular) {
                 // difference-form Esirkepov weights of a particle that stays in its
-                // cell (Esirkepov.py:158-214): W = D_c (S0 S0 + (S0 D + D S0)/2 + D D/3)
-                // over the other two axes, J_c = -(q/(d_o1 d_o2 dt)) W at slot a only
+                // cell (Esirkepov.py:158-214): for component c over the other axes p, q,
+                // W = D_c (S0p S0q + (S0p dq + dp S0q)/2 + dp dq/3) with d = -D, +D at
+                // slots 0, 1, i.e. W = D_c (Ap Aq + dp dq/12) with A = S0 + d/2;
+                // J_c = -(q/(d_p d_q dt)) W at slot a only
+                float A[3][2];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    A[a][1] = S0[a][1] + 0.5f * D[a];
+                    A[a][0] = S0[a][0] - 0.5f * D[a];
+                }
                 float val[12];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const int o1 = c == 0 ? 1 : 0, o2 = c == 2 ? 1 : 2;
                     const float pre = k.pre[c] * (-D[c]);
-#pragma unroll
-                    for (int jj = 0; jj < 2; ++jj) {
-                        const float dj = jj ? D[o1] : -D[o1];
-#pragma unroll
-                        for (int kk = 0; kk < 2; ++kk) {
-                            const float dk = kk ? D[o2] : -D[o2];
-                            const float y = S0[o1][jj] * S0[o2][kk] + 0.5f * (S0[o1][jj] * dk + dj * S0[o2][kk]) +
-                                            (1.0f / 3.0f) * dj * dk;
-                            val[c * 4 + jj * 2 + kk] = pre * y;
-                        }
-                    }
+                    const float e = D[o1] * D[o2] * (1.0f / 12.0f);
+                    val[c * 4 + 0] = pre * fmaf(A[o1][0], A[o2][0], e);
+                    val[c * 4 + 1] = pre * fmaf(A[o1][0], A[o2][1], -e);
+                    val[c * 4 + 2] = pre * fmaf(A[o1][1], A[o2][0], -e);
+                    val[c * 4 + 3] = pre * fmaf(A[o1][1], A[o2][1], e);
                 }
                 const bool same = have && anc[0] == h0 && anc[1] == h1 && anc[2] == h2;
                 if (have && !same) flush_cell<kB>(d, sink, h0, h1, h2, acc);
@@ -415,7 +486,10 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
             // ---- position update + particle boundaries (particle_tile_communication.py)
             bool act = true;
 #pragma unroll
-            for (int a = 0; a < 3; ++a) apply_axis_boundary(d.pbc[a], d.wind[a], xn[a], v[a], act);
+            for (int a = 0; a < 3; ++a) {
+                if (d.pbc[a] == 0) periodic_wrap(d.wind[a], xn[a]);
+                else apply_axis_boundary(d.pbc[a], d.wind[a], xn[a], v[a], act);
+            }
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 out.x[a][i] = act ? (float)xn[a] : 0.0f;
@@ -438,8 +512,9 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
             bool ok = cross_axis(xo[0], xn[0], d.org_c[0], d.sp_c[0], 1.0 / d.d[0], eg.org_c[0], eg.isp_c[0], cx);
             ok = cross_axis(xo[1], xn[1], d.org_c[1], d.sp_c[1], 1.0 / d.d[1], eg.org_c[1], eg.isp_c[1], cy) && ok;
             ok = cross_axis(xo[2], xn[2], d.org_c[2], d.sp_c[2], 1.0 / d.d[2], eg.org_c[2], eg.isp_c[2], cz) && ok;
-            if (ok) crossing_deposit(d, k.qw, cx, cy, cz, sink);
-            else rare_push(rare, i);     // not reached after the step-length test; the rare path redoes it whole
+            // a frame outside the box, or a move of two nodes (excluded by the step-length
+            // test): the rare kernel redoes the particle whole (same outputs)
+            if (!(ok && crossing_deposit_box<kB>(k, cx, cy, cz, sink))) rare_push(rare, i);
         }
         __syncthreads();
     }
@@ -472,29 +547,27 @@ __global__ void zero_count_kernel(int32_t* c) { *c = 0; }
 // Host check for ExactGrid (see its comment); false when any axis fails.
 bool make_exact_grid(const PicDesc& d, ExactGrid& eg) {
     for (int a = 0; a < 3; ++a) {
-        const double org[2] = {d.org_c[a], d.org_v[a]}, spc[2] = {d.sp_c[a], d.sp_v[a]};
+        const double org = d.org_c[a], sp = d.sp_c[a];
+        const float o32 = (float)org, s32 = (float)sp, h32 = 0.5f * s32;
+        if ((double)o32 != org || (double)s32 != sp || !(sp > 0.0)) return false;
+        if (d.sp_v[a] != sp || d.org_v[a] != org + 0.5 * sp || (double)(o32 + h32) != d.org_v[a]) return false;
+        if (d.dims[a] > (1 << 20)) return false;
         const int lo = -d.dims[a] - 4, hi = 2 * d.dims[a] + 4;
-        for (int gl = 0; gl < 2; ++gl) {
-            const float o32 = (float)org[gl], s32 = (float)spc[gl];
-            if ((double)o32 != org[gl] || (double)s32 != spc[gl] || !(spc[gl] > 0.0)) return false;
-            for (int n = lo; n <= hi + 1; ++n) {
-                const double node = (double)n * spc[gl] + org[gl];          // exact: 24-bit operands
-                const float n32 = std::fma((float)n, s32, o32);
-                if ((double)n32 != node) return false;
-            }
+        for (int n = lo - 1; n <= hi + 1; ++n) {
+            const double node = (double)n * sp + org;                   // exact: 24-bit operands
+            const float n32 = std::fma((float)n, s32, o32);
+            if ((double)n32 != node || (double)(n32 + h32) != node + 0.5 * sp) return false;
         }
-        eg.org_c[a] = (float)d.org_c[a];
-        eg.sp_c[a] = (float)d.sp_c[a];
-        eg.isp_c[a] = (float)(1.0 / d.sp_c[a]);
-        eg.org_v[a] = (float)d.org_v[a];
-        eg.sp_v[a] = (float)d.sp_v[a];
-        eg.isp_v[a] = (float)(1.0 / d.sp_v[a]);
+        eg.org_c[a] = o32;
+        eg.sp_c[a] = s32;
+        eg.isp_c[a] = (float)(1.0 / sp);
+        eg.half_sp[a] = h32;
         eg.inv_d[a] = (float)(1.0 / d.d[a]);
-        eg.lo[a] = lo;
-        eg.hi[a] = hi;
+        eg.xlo[a] = std::fma((float)lo, s32, o32);
+        eg.xhi[a] = std::fma((float)hi, s32, o32);
         // floor((x - org)/sp) in FP64 can differ from the exact bracket only within a
-        // few FP64 ulps of the quotient (|quotient| < 2^31): a band of 2^-20 sp is ample
-        eg.tol[a] = std::ldexp(d.sp_c[a], -20);
+        // few FP64 ulps of the quotient (|quotient| < 2^22 here): a band of 2^-20 sp is ample
+        eg.tol[a] = std::ldexp(sp, -20);
     }
     return true;
 }
