@@ -4,15 +4,19 @@ Workload (default): periodic 256^3 grid, warm isotropic electron plasma at
 vth = 0.05c, 32 particles per cell (536,870,912 particles, SoA FP32), first
 order (CIC) Esirkepov current, relativistic Boris, Yee B/2 -> E -> B/2,
 omega_p dt = 0.1, cfl 0.5.  A step is one time_loop_electrodynamic call
-(evolve.py:16-103) executed by the allocation-free runner, with a stable cell
-re-sort every --sort-every steps (counted in the timed region).
+(evolve.py:16-103) executed by the allocation-free runner on tile-sorted
+particles (4^3-cell tiles by default), with a stable tile re-sort every
+--sort-every steps (counted in the timed region).
 
 Prints ONE JSON line.  value = particle-steps/s over the timed steps.
 roofline: the fused particle kernel (gather + Boris + Esirkepov + position
 update + boundaries), algorithmic bytes 50 per particle (read x,y,z,vx,vy,vz
 and the active byte, write them back) + 36 per cell (E, B gather reads and the
-J write), divided by its mean launch duration from HIP events recorded on the
-stream the kernel runs on.
+J write), divided by the mean duration of the particle phase from HIP events
+recorded on the stream the kernel runs on (the phase is the fused kernel plus
+its rare-list kernel, about 0.15 ms).  traffic: calibrated FETCH_SIZE +
+WRITE_SIZE bytes per launch from the kernel's committed rocprofv3 summary in
+profiles/ for this workload (null when there is none).
 cpu_baseline: the FP64 NumPy oracle (oracle/, one core) on a bounded sample.
 """
 
@@ -28,6 +32,18 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/s (push+Esirkepov+Yee) 256³·32ppc; HBM GB/s; adjoint/fwd cost"
+
+
+def measured_traffic(kernel, n, ppc):
+    """HBM bytes per launch of `kernel` on this workload from its committed
+    rocprofv3 summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{n}cubed_{ppc}ppc.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1].split()[0]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def cpu_baseline(n=32, ppc=16):
@@ -325,6 +341,7 @@ def main():
         kernel_name = "fused_tile_exact_kernel" if exact else "fused_tiled_kernel"
     else:
         kernel_name = "fused_push_esirkepov_v2"
+    traffic, traffic_src = measured_traffic(kernel_name, args.n, args.ppc)
     ms_per_step = 1e3 * elapsed / args.steps
     value = n_particles * args.steps / elapsed
     kern_ms = phases.get("particles")
@@ -356,7 +373,8 @@ def main():
             "kernel": kernel_name,
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None, "kernel_ms": kern_ms, "algorithmic_bytes": kern_bytes,
+            "traffic": traffic, "traffic_source": traffic_src, "kernel_ms": kern_ms,
+            "algorithmic_bytes": kern_bytes,
         },
         "step_hbm_gbs": step_bytes * args.steps / elapsed / 1e9,
         "phase_ms": phases,
