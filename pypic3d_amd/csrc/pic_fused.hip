@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tiled_kernel(PicDe
 constexpr int kCrossCap = 512;
 struct CrossQueue {
     int q_idx[kCrossCap];
-    float q_v[3][kCrossCap];
+    float q_f[6][kCrossCap];
     int q_n;
 };
 
@@ -264,9 +264,55 @@ __device__ inline void tile_particle_general(const PicDesc& d, const SpeciesK& k
 // rare[1]) without being stored; fused_tile_rare_kernel then runs it whole.
 // Keeping those paths out of this kernel keeps its register count low.
 
+template <int kP>
+__device__ inline void store_vec(float* p, const float v[kP]) {
+    if constexpr (kP == 4) *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    else *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+}
+
 __device__ inline void rare_push(int32_t* rare, int64_t i) {
     const int p = atomicAdd(rare, 1);
     rare[1 + p] = (int32_t)i;
+}
+
+// 3-slot crossing frame of one axis (the values cross_axis computes in FP64)
+// from the quantities the fast loop queues: the anchor step st = a_new -
+// a_old in {-1, 0, +1}, ro = (x_old - node_old)/d and f = (x_new -
+// node_new)/d, or the step itself v dt/d when st = 0 (no cancellation then).
+// rel is the old anchor relative to the box origin; b becomes the frame's
+// first node relative to it.
+__device__ inline void cross_frame(int rel, int st, float ro, float f, CrossAxis& e) {
+    if (st == 0) {                    // frame (a, a+1, -): S0 = (1-ro, ro), D = (-dl, dl)
+        e.b = rel;
+        e.S0[0] = 1.0f - ro;
+        e.S0[1] = ro;
+        e.S0[2] = 0.0f;
+        e.D[0] = -f;
+        e.D[1] = f;
+        e.D[2] = 0.0f;
+        e.C0 = -f;
+        e.C1 = 0.0f;
+    } else if (st > 0) {              // frame (a, a+1, a+2): S0 = (1-ro, ro, 0), S1 = (0, 1-rn, rn)
+        e.b = rel;
+        e.S0[0] = 1.0f - ro;
+        e.S0[1] = ro;
+        e.S0[2] = 0.0f;
+        e.D[0] = ro - 1.0f;
+        e.D[2] = f;
+        e.D[1] = -(e.D[0] + f);
+        e.C0 = e.D[0];
+        e.C1 = -f;
+    } else {                          // frame (a-1, a, a+1): S0 = (0, 1-ro, ro), S1 = (1-rn, rn, 0)
+        e.b = rel - 1;
+        e.S0[0] = 0.0f;
+        e.S0[1] = 1.0f - ro;
+        e.S0[2] = ro;
+        e.D[0] = 1.0f - f;
+        e.D[2] = -ro;
+        e.D[1] = -(e.D[0] - ro);
+        e.C0 = e.D[0];
+        e.C1 = ro;
+    }
 }
 
 // Face-crossing deposit (the 54 terms of crossing_deposit) straight into the
@@ -297,24 +343,21 @@ __device__ inline void crossing_component_box(const float pre, const CrossAxis& 
     }
 }
 
-// Returns false (nothing deposited) when the frame leaves the box.
+// Frames given relative to the box origin; the caller has checked that
+// every frame base lies in [0, kB - 3].
 template <int kB>
-__device__ inline bool crossing_deposit_box(const SpeciesK& k, const CrossAxis& cx, const CrossAxis& cy,
-                                            const CrossAxis& cz, const BoxSink& sink) {
-    const int bi = cx.b - sink.o[0], bj = cy.b - sink.o[1], bk = cz.b - sink.o[2];
-    if ((unsigned)bi > (unsigned)(kB - 3) || (unsigned)bj > (unsigned)(kB - 3) || (unsigned)bk > (unsigned)(kB - 3))
-        return false;
-    float* p = sink.box + (bi * kB + bj) * kB + bk;
+__device__ inline void crossing_deposit_box(const SpeciesK& k, const CrossAxis& cx, const CrossAxis& cy,
+                                            const CrossAxis& cz, float* box) {
+    float* p = box + (cx.b * kB + cy.b) * kB + cz.b;
     crossing_component_box<0, kB>(k.pre[0], cx, cy, cz, p);
     crossing_component_box<1, kB>(k.pre[1], cy, cx, cz, p);
     crossing_component_box<2, kB>(k.pre[2], cz, cx, cy, p);
-    return true;
 }
 
 template <int kT, int kP, int kMinBlocks>
 __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
     PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, PicParticles out, float* Jx, float* Jy, float* Jz,
-    const int32_t* __restrict__ starts, TileGeom tg, ExactGrid eg, int32_t* __restrict__ rare) {
+    const int32_t* __restrict__ starts, TileGeom tg, ExactGrid eg, int32_t* __restrict__ rare, int vec_ok) {
     constexpr int kB = kT + 4;
     constexpr int kB3 = kB * kB * kB;
     constexpr int kFB = kT + 4;
@@ -364,23 +407,24 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
         for (int m = 0; m < 12; ++m) acc[m] = 0.0f;
         int h0 = 0, h1 = 0, h2 = 0;
         bool have = false;
-        // one particle per iteration; the loop stays rolled to bound register pressure
+        // One particle per iteration (the loop stays rolled to bound register
+        // pressure).  Outputs shift through OX/OU/oact so that a group lying
+        // inside the range is stored with one kP-wide vector per plane; slots
+        // sent to the rare list are stored as zeros here and rewritten by the
+        // rare kernel.
+        float OX[3][kP], OU[3][kP];
+        uint32_t oact = 0u;
 #pragma unroll 1
         for (int j = 0; j < kP; ++j) {
             const int64_t i = s0 + j;
-            if (i < begin || i >= end) continue;
+            float ox[3] = {0.0f, 0.0f, 0.0f}, ou[3] = {0.0f, 0.0f, 0.0f};
+            uint32_t oa = 0u;
+            do {
+            if (i < begin || i >= end) break;
             const bool act0 = in.active[i] != 0;
             const float xf[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
             float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
-            if (!act0) {
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    out.x[a][i] = 0.0f;
-                    out.u[a][i] = 0.0f;
-                }
-                out.active[i] = 0;
-                continue;
-            }
+            if (!act0) break;
             // ---- anchors on both node lines (grid_and_stencil.py:92-166), exact in FP32
             int anc[3], anv[3];
             float dc[3], dv[3], nodec[3];
@@ -395,7 +439,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
             }
             if (!fast) {
                 rare_push(rare, i);
-                continue;
+                break;
             }
             // ---- gather (boris.py:302-433) from the tile's field box
             CicAxis gc[3], gv[3];
@@ -441,15 +485,37 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
                 D[a] = dvt * eg.inv_d[a];
             }
             if (!regular) {
-                const int slot = short_step ? atomicAdd(&sh.q_n, 1) : kCrossCap;
+                // face crossing: the new anchor by bracketing the FP64 new position
+                // between the exact nodes next to the old cell (outside the face band)
+                // and a frame inside the box, else the rare list; the crossing pass
+                // rebuilds the frame from (step, ro, f, old anchor)
+                bool ok = short_step;
+                int packed = kP * threadIdx.x + j;
+                float fro[3], ff[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float hi32 = nodec[a] + eg.sp_c[a];
+                    const int st = xn[a] < (double)nodec[a] ? -1 : (xn[a] >= (double)hi32 ? 1 : 0);
+                    const float noden = st == 0 ? nodec[a] : (st < 0 ? nodec[a] - eg.sp_c[a] : hi32);
+                    ok = ok && xn[a] >= (double)noden + eg.tol[a] &&
+                         xn[a] < (double)(noden + eg.sp_c[a]) - eg.tol[a];
+                    const int rel = anc[a] - o[a];
+                    ok = ok && (unsigned)(rel + (st < 0 ? -1 : 0)) <= (unsigned)(kB - 3);
+                    fro[a] = dc[a] * eg.inv_d[a];
+                    ff[a] = st == 0 ? D[a] : (float)(xn[a] - (double)noden) * eg.inv_d[a];
+                    packed |= ((st + 1) << (10 + 2 * a)) | ((rel & 31) << (16 + 5 * a));
+                }
+                const int slot = ok ? atomicAdd(&sh.q_n, 1) : kCrossCap;
                 if (slot >= kCrossCap) {
                     rare_push(rare, i);
-                    continue;
+                    break;
                 }
-                sh.q_idx[slot] = kP * threadIdx.x + j;
-                sh.q_v[0][slot] = v[0];
-                sh.q_v[1][slot] = v[1];
-                sh.q_v[2][slot] = v[2];
+                sh.q_idx[slot] = packed;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    sh.q_f[a][slot] = fro[a];
+                    sh.q_f[3 + a][slot] = ff[a];
+                }
             }
             if (regular) {
                 // difference-form Esirkepov weights of a particle that stays in its
@@ -492,29 +558,57 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
             }
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                out.x[a][i] = act ? (float)xn[a] : 0.0f;
-                out.u[a][i] = act ? v[a] : 0.0f;
+                ox[a] = act ? (float)xn[a] : 0.0f;
+                ou[a] = act ? v[a] : 0.0f;
             }
-            out.active[i] = act ? 1 : 0;
+            oa = act ? 1u : 0u;
+            } while (false);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+#pragma unroll
+                for (int q = 0; q + 1 < kP; ++q) {
+                    OX[a][q] = OX[a][q + 1];
+                    OU[a][q] = OU[a][q + 1];
+                }
+                OX[a][kP - 1] = ox[a];
+                OU[a][kP - 1] = ou[a];
+            }
+            oact = (oact >> 8) | (oa << (8 * (kP - 1)));
+        }
+        const bool full = vec_ok && s0 >= begin && s0 + kP <= end;
+        if (full) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                store_vec<kP>(out.x[a] + s0, OX[a]);
+                store_vec<kP>(out.u[a] + s0, OU[a]);
+            }
+            if constexpr (kP == 4) *reinterpret_cast<uint32_t*>(out.active + s0) = oact;
+            else *reinterpret_cast<uint16_t*>(out.active + s0) = (uint16_t)oact;
+        } else {
+#pragma unroll
+            for (int j = 0; j < kP; ++j) {
+                const int64_t i = s0 + j;
+                if (i < begin || i >= end) continue;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    out.x[a][i] = OX[a][j];
+                    out.u[a][i] = OU[a][j];
+                }
+                out.active[i] = (uint8_t)((oact >> (8 * j)) & 0xffu);
+            }
         }
         if (have) flush_cell<kB>(d, sink, h0, h1, h2, acc);
         __syncthreads();
         // ---- queued face crossings (3-slot frame, Esirkepov.py:105-331)
         const int nq = min(sh.q_n, kCrossCap);
         for (int q = threadIdx.x; q < nq; q += kThreads) {
-            const int64_t i = base + sh.q_idx[q];
-            const float v[3] = {sh.q_v[0][q], sh.q_v[1][q], sh.q_v[2][q]};
-            double xo[3], xn[3];
-            load_pos(in, i, xo);
+            const int e = sh.q_idx[q];
+            CrossAxis ca[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) xn[a] = ((k.ux >> a) & 1u) ? step_pos(xo[a], (double)v[a], d.dt) : xo[a];
-            CrossAxis cx, cy, cz;
-            bool ok = cross_axis(xo[0], xn[0], d.org_c[0], d.sp_c[0], 1.0 / d.d[0], eg.org_c[0], eg.isp_c[0], cx);
-            ok = cross_axis(xo[1], xn[1], d.org_c[1], d.sp_c[1], 1.0 / d.d[1], eg.org_c[1], eg.isp_c[1], cy) && ok;
-            ok = cross_axis(xo[2], xn[2], d.org_c[2], d.sp_c[2], 1.0 / d.d[2], eg.org_c[2], eg.isp_c[2], cz) && ok;
-            // a frame outside the box, or a move of two nodes (excluded by the step-length
-            // test): the rare kernel redoes the particle whole (same outputs)
-            if (!(ok && crossing_deposit_box<kB>(k, cx, cy, cz, sink))) rare_push(rare, i);
+            for (int a = 0; a < 3; ++a)
+                cross_frame((e >> (16 + 5 * a)) & 31, ((e >> (10 + 2 * a)) & 3) - 1, sh.q_f[a][q], sh.q_f[3 + a][q],
+                            ca[a]);
+            crossing_deposit_box<kB>(k, ca[0], ca[1], ca[2], box);
         }
         __syncthreads();
     }
@@ -666,10 +760,15 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
     ExactGrid eg;
     if (desc->variant != 1 && make_exact_grid(*desc, eg)) {
         PIC_REQUIRE(workspace);
+        // vector stores of a thread's slot group need 16-byte aligned planes
+        int vec_ok = 1;
+        for (int a = 0; a < 3; ++a)
+            vec_ok &= ((((uintptr_t)out->x[a]) | ((uintptr_t)out->u[a])) & 15u) == 0;
+        vec_ok &= (((uintptr_t)out->active) & 3u) == 0;
         hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, workspace);
 #define PIC_EXACT_LAUNCH(T, P, MB)                                                                                  \
         hipLaunchKernelGGL((fused_tile_exact_kernel<T, P, MB>), dim3((unsigned)blocks), dim3(kThreads), 0, st,      \
-                           *desc, *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace)
+                           *desc, *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace, vec_ok)
 #define PIC_EXACT_MB(T, P)                                  \
         do {                                                \
             if (min_blocks == 3) PIC_EXACT_LAUNCH(T, P, 3); \
