@@ -241,6 +241,82 @@ __device__ inline void push_t(const T v[3], const T E[3], const T B[3], double h
     }
 }
 
+// Reverse mode of the (relativistic) Boris push of push_t, in FP64: given
+// lo = dL/d(v'), returns lin = dL/d(v, E_p, B_p) in one backward sweep over
+// the forward values (the tangent passes of push_t take nine sweeps).  Each
+// line below is the transpose of one forward statement, in reverse order;
+// for c = a x b the transposes are la += b x lc, lb += lc x a.
+__device__ inline void boris_vjp(const double v[3], const double E[3], const double B[3], double h,
+                                 int relativistic, double C, const double lo[3], double lin[9]) {
+    const double C2 = C * C;
+    double um[3], t[3], up[3], s[3], c1[3], nu[3];
+    double g = 1.0, gm = 1.0;
+    if (relativistic) {
+        g = 1.0 / sqrt(1.0 - tdot(v, v) / C2);
+        for (int a = 0; a < 3; ++a) um[a] = v[a] * g + E[a] * h;
+        gm = sqrt(1.0 + tdot(um, um) / C2);
+        for (int a = 0; a < 3; ++a) t[a] = B[a] * h / gm;
+    } else {
+        for (int a = 0; a < 3; ++a) {
+            um[a] = v[a] + E[a] * h;
+            t[a] = B[a] * h;
+        }
+    }
+    tcross(um, t, c1);
+    for (int a = 0; a < 3; ++a) up[a] = um[a] + c1[a];
+    const double tt = 1.0 + tdot(t, t);
+    for (int a = 0; a < 3; ++a) s[a] = 2.0 * t[a] / tt;
+    tcross(up, s, c1);
+    for (int a = 0; a < 3; ++a) nu[a] = um[a] + c1[a] + E[a] * h;
+    // out = nu / ng
+    double lnu[3];
+    if (relativistic) {
+        const double ng = sqrt(1.0 + tdot(nu, nu) / C2);
+        const double k = tdot(lo, nu) / (C2 * ng * ng * ng);
+        for (int a = 0; a < 3; ++a) lnu[a] = lo[a] / ng - k * nu[a];
+    } else {
+        for (int a = 0; a < 3; ++a) lnu[a] = lo[a];
+    }
+    // nu = um + c2 + E h,  c2 = up x s
+    double lum[3], lE[3], lup[3], ls[3], lt[3];
+    for (int a = 0; a < 3; ++a) {
+        lum[a] = lnu[a];
+        lE[a] = h * lnu[a];
+    }
+    tcross(s, lnu, lup);
+    tcross(lnu, up, ls);
+    // s = 2 t / tt
+    const double st = tdot(ls, t);
+    for (int a = 0; a < 3; ++a) lt[a] = 2.0 * ls[a] / tt - 4.0 * t[a] * st / (tt * tt);
+    // up = um + c1,  c1 = um x t
+    double w[3];
+    tcross(t, lup, w);
+    for (int a = 0; a < 3; ++a) lum[a] += lup[a] + w[a];
+    tcross(lup, um, w);
+    for (int a = 0; a < 3; ++a) lt[a] += w[a];
+    double lv[3];
+    if (relativistic) {
+        // t = B h / gm,  gm = sqrt(1 + um.um / C2)
+        const double lgm = -tdot(t, lt) / gm;
+        for (int a = 0; a < 3; ++a) {
+            lin[6 + a] = h * lt[a] / gm;
+            lum[a] += lgm * um[a] / (C2 * gm);
+        }
+        // um = v g + E h,  g = (1 - v.v / C2)^(-1/2)
+        const double lg = tdot(v, lum);
+        for (int a = 0; a < 3; ++a) lv[a] = g * lum[a] + lg * g * g * g * v[a] / C2;
+    } else {
+        for (int a = 0; a < 3; ++a) {
+            lin[6 + a] = h * lt[a];
+            lv[a] = lum[a];
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        lin[a] = lv[a];
+        lin[3 + a] = lE[a] + h * lum[a];
+    }
+}
+
 // ----------------------------------------------------------------- gather
 // One axis of interpolate_field_to_particles (boris.py:127-258): stencil
 // points, weights and their x-derivatives (weights divide by the grid spacing).
@@ -638,7 +714,9 @@ __device__ inline void gather_values(const PicDesc& d, const FieldPtrs& fp, cons
 // cells on an axis is appended to `defer` (untouched here) for the general
 // pass.  !kAnalytic: the general dual-number path, over all slots (only ==
 // nullptr) or over the `*count` slots listed in `only`.
-template <bool kAnalytic, int kMinBlocks>
+// kPush: 1 = Boris reverse sweep only, 2 = tangent passes only, 0 = chosen
+// per launch from the descriptor (both compiled in).
+template <bool kAnalytic, int kMinBlocks, int kPush = 0>
 __global__ __launch_bounds__(kAdjThreads, kMinBlocks) void adjoint_particles_kernel(
     PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, LamPtrs lp, int64_t n, int32_t* defer,
     int32_t* defer_count, const int32_t* only, const int32_t* only_count) {
@@ -719,6 +797,14 @@ __global__ __launch_bounds__(kAdjThreads, kMinBlocks) void adjoint_particles_ker
     }
     // ---- phase 2: lambda(v, E_p, B_p) = J_push^T lambda(v') on update_u axes
     double lin[9];
+    const bool vjp = kPush == 1 || (kPush == 0 && d.pusher == 0 && d.variant != 1);
+    if (kPush != 2 && vjp) {
+        double lo[3];
+        for (int a = 0; a < 3; ++a) lo[a] = ((k.uu >> a) & 1u) ? lv[a] : 0.0;
+        boris_vjp(v, Ep, Bp, h, d.relativistic, d.C, lo, lin);
+        for (int a = 0; a < 3; ++a)
+            if (!((k.uu >> a) & 1u)) lin[a] += lv[a];        // frozen component: v' = v
+    } else
 #pragma unroll 1
     for (int m = 0; m < 9; ++m) {
         Dual<1> vv[3], EE[3], BB[3], out[3];
@@ -1001,15 +1087,21 @@ int pic_adjoint_particles(const PicDesc* desc, const PicSpecies* sp, const PicPa
         return 0;
     }
     hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, count);
-    if (min_blocks >= 8)
-        hipLaunchKernelGGL((adjoint_particles_kernel<true, 8>), grid, blk, 0, st, *desc, *sp, *state, fp, lp, n, list,
-                           count, nullptr, nullptr);
-    else if (min_blocks >= 4)
-        hipLaunchKernelGGL((adjoint_particles_kernel<true, 4>), grid, blk, 0, st, *desc, *sp, *state, fp, lp, n, list,
-                           count, nullptr, nullptr);
-    else
-        hipLaunchKernelGGL((adjoint_particles_kernel<true, 1>), grid, blk, 0, st, *desc, *sp, *state, fp, lp, n, list,
-                           count, nullptr, nullptr);
+    const bool vjp = desc->pusher == 0 && desc->variant != 1;
+#define PIC_ADJ_LAUNCH(MB, PUSH)                                                                                  \
+    hipLaunchKernelGGL((adjoint_particles_kernel<true, MB, PUSH>), grid, blk, 0, st, *desc, *sp, *state, fp, lp, n, \
+                       list, count, nullptr, nullptr)
+    if (min_blocks >= 8) {
+        if (vjp) PIC_ADJ_LAUNCH(8, 1);
+        else PIC_ADJ_LAUNCH(8, 2);
+    } else if (min_blocks >= 4) {
+        if (vjp) PIC_ADJ_LAUNCH(4, 1);
+        else PIC_ADJ_LAUNCH(4, 2);
+    } else {
+        if (vjp) PIC_ADJ_LAUNCH(1, 1);
+        else PIC_ADJ_LAUNCH(1, 2);
+    }
+#undef PIC_ADJ_LAUNCH
     hipLaunchKernelGGL((adjoint_particles_kernel<false, 1>), grid2, blk, 0, st, *desc, *sp, *state, fp, lp, n, nullptr,
                        nullptr, list, count);
     PIC_CHECK_LAUNCH();

@@ -165,3 +165,29 @@ def test_gradients_do_not_depend_on_the_checkpoint_interval():
         assert abs(got[1] - ref[1]) <= 1e-6 * abs(ref[1])
         assert abs(got[2] - ref[2]) <= 1e-6 * abs(ref[2])
         assert np.max(np.abs(got[3] - ref[3])) <= 1e-6 * np.max(np.abs(ref[3]))
+
+
+@pytest.mark.parametrize("relativistic", [True, False])
+def test_reverse_mode_push_matches_the_tangent_passes(relativistic):
+    """The Boris push adjoint in one reverse sweep (default) against the nine
+    forward-mode tangent passes (desc.variant = 1).  The two runs differ also
+    by the FP32 atomic summation order of their recomputed forward steps
+    (~1e-7), so the bound is 1e-5 of the largest gradient entry; a wrong term
+    in either form is an O(1) difference."""
+    static, dyn = workloads.make_parameters((12, 10, 1), cfl=0.5, relativistic=relativistic)
+    _, _, cfg = _setup()
+    grads = []
+    for variant in (0, 1):
+        x0, v0 = two_stream_state(static, dyn, 2, 0.1, 0.05, vth=0.01, seed=5, device="cpu")
+        g = int(static.guard_cells)
+        shape = (3, int(dyn.Nx) + 2 * g, int(dyn.Ny) + 2 * g, int(dyn.Nz) + 2 * g)
+        rng = np.random.default_rng(11)
+        E0 = torch.tensor(0.05 * rng.standard_normal(shape), dtype=torch.float32, device="cuda")
+        B0 = torch.tensor(0.05 * rng.standard_normal(shape), dtype=torch.float32, device="cuda")
+        act = torch.ones((2, x0.shape[2]), dtype=torch.bool, device="cuda")
+        run = AdjointRun(static, dyn, cfg, act, checkpoint_every=4)
+        run.forward(x0.cuda(), v0.cuda(), E0, B0, 6)
+        run.runner.desc.variant = variant
+        grads.append([t.cpu().numpy() for t in run.backward()])
+    for a, b in zip(*grads):
+        assert np.max(np.abs(a - b)) <= 1e-5 * max(np.max(np.abs(b)), 1e-30)
