@@ -24,6 +24,8 @@ state of the forward kernels.
 
 from __future__ import annotations
 
+import ctypes
+
 import math
 
 import torch
@@ -46,7 +48,7 @@ class AdjointRun:
     """Forward with checkpoints, then reverse sweep with recomputation."""
 
     def __init__(self, static_parameters, dynamic_parameters, species_config, active, external_fields=None,
-                 checkpoint_every=20, min_blocks=4):
+                 checkpoint_every=20, min_blocks=8):
         _check_supported(static_parameters, dynamic_parameters)
         self.static, self.dynamic = static_parameters, dynamic_parameters
         self.cfg = species_config
@@ -138,9 +140,19 @@ class AdjointRun:
         gshape = tuple(r.J[0].shape)
         seg_x = torch.empty((k, 3, r.S, r.P), dtype=torch.float32, device=dev)
         seg_v = torch.empty_like(seg_x)
-        seg_a = torch.empty((k, r.S, r.P), dtype=torch.bool, device=dev)
+        # periodic particle boundaries never change the active set: one copy serves every step
+        seg_a = r.act[r.cur].clone()
         seg_E = torch.empty((k, 3) + gshape, dtype=torch.float32, device=dev)
         seg_B = torch.empty_like(seg_E)
+        # C-ABI arguments of every segment slot, built once (the reverse sweep is
+        # launch-bound at small sizes)
+        lib = N.lib()
+        seg_pp = [ctypes.byref(pic_particles([seg_x[m, 0], seg_x[m, 1], seg_x[m, 2]],
+                                             [seg_v[m, 0], seg_v[m, 1], seg_v[m, 2]], seg_a)) for m in range(k)]
+        seg_pE = [p3(seg_E[m]) for m in range(k)]
+        seg_pB = [p3(seg_B[m]) for m in range(k)]
+        dref, spref, dfr = ctypes.byref(d), ctypes.byref(r.sp), N.ptr(defer)
+        adj_yee, adj_part = lib.pic_adjoint_yee, lib.pic_adjoint_particles
         starts = sorted(self.ckpt)
         for si in reversed(range(len(starts))):
             n0 = starts[si]
@@ -149,16 +161,13 @@ class AdjointRun:
             for m in range(n1 - n0):
                 seg_x[m].copy_(r.pos[r.cur])
                 seg_v[m].copy_(r.vel[r.cur])
-                seg_a[m].copy_(r.act[r.cur])
                 torch.stack(r.E[r.ecur], out=seg_E[m])
                 torch.stack(r.B, out=seg_B[m])
                 r.step()
             for m in reversed(range(n1 - n0)):
-                N.call("pic_adjoint_yee", d, LE, LB, LJ, w6, s)
-                pp = pic_particles([seg_x[m, 0], seg_x[m, 1], seg_x[m, 2]], [seg_v[m, 0], seg_v[m, 1], seg_v[m, 2]],
-                                   seg_a[m])
-                N.call("pic_adjoint_particles", d, r.sp, pp, p3(seg_E[m]), p3(seg_B[m]), r.Ee, r.Be, LJ, LX, LV, LE,
-                       LB, self.min_blocks, N.ptr(defer), s)
+                N.check(adj_yee(dref, LE, LB, LJ, w6, s), "pic_adjoint_yee")
+                N.check(adj_part(dref, spref, seg_pp[m], seg_pE[m], seg_pB[m], r.Ee, r.Be, LJ, LX, LV, LE, LB,
+                                 self.min_blocks, dfr, s), "pic_adjoint_particles")
         return lx, lv, lamE, lamB
 
 
