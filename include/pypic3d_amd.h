@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define PIC_ABI_VERSION 3
+#define PIC_ABI_VERSION 4
 #define PIC_MAX_SPECIES 16
 
 /* Scalar run description: StaticParameters + DynamicParameters fields the hot
@@ -100,6 +100,26 @@ int pic_yee_update_b(const PicDesc* desc, const float* const E[3], const float* 
  * then E_out ghosts refreshed.  Filter and conducting walls are separate calls. */
 int pic_yee_update_e(const PicDesc* desc, const float* const E_in[3], const float* const B[3],
                      const float* const J[3], float* const E_out[3], void* stream);
+
+/* PML (boundary_conditions/PML.py): ADE memory of the six curl derivatives
+ * (interior (Nx, Ny, Nz) float32 arrays, updated in place; order dEz/dy,
+ * dEy/dz, dEx/dz, dEz/dx, dEy/dx, dEx/dy for B and the same with B for E)
+ * and the damping profiles sigma_x, sigma_y, sigma_z in the field layout. */
+typedef struct PicPml {
+    float* mem[6];
+    const float* sigma[3];
+} PicPml;
+/* pic_yee_update_b / _e with apply_tiled_pml_to_b_curl / _e_curl
+ * (PML.py:270-359; first_order_yee.py:54-59, 133-138) applied to the
+ * derivatives: b = exp(-sigma dt) with the full dt, m' = b m + (b-1) d,
+ * d' = d + m'. */
+int pic_yee_update_b_pml(const PicDesc* desc, const float* const E[3], const float* const B_in[3],
+                         float* const B_out[3], const PicPml* pml, void* stream);
+int pic_yee_update_e_pml(const PicDesc* desc, const float* const E_in[3], const float* const B[3],
+                         const float* const J[3], float* const E_out[3], const PicPml* pml, void* stream);
+/* stretch_spatial_derivative on six interior derivative arrays in place, in
+ * the memory order above (the derivative part of apply_tiled_pml_to_*_curl). */
+int pic_pml_stretch(const PicDesc* desc, float* const deriv[6], const PicPml* pml, void* stream);
 
 /* ---- particles --------------------------------------------------------- */
 /* particle_push (pusher/particle_push.py:13-175): gather E+E_ext, B+B_ext on

@@ -32,6 +32,12 @@ Parity pins (checks that do not retype the formula; tests/test_oracle_*.py):
 * sort: ``np.argsort(kind="stable")`` is the specification itself
   (particle_initialization.py:159); checked against a brute-force bucket sort.
 * energy: closed forms on uniform fields / single particles.
+* PML: profile values at hand-computed indices and the validation errors of
+  PML.py:17-81; the ADE memory under a constant derivative follows the closed
+  form m_n = (b^n - 1) d, so the stretched derivative decays as b^n d;
+  sigma = 0 leaves the Yee updates unchanged; a 1-D pulse between PML walls
+  loses most of its energy (pml_test.py:527-617 setup) while conducting walls
+  alone keep it.
 """
 
 from __future__ import annotations
@@ -655,8 +661,96 @@ def compute_rho(x, active, sp: Species, p: Params):
 # Yee solver (solvers/first_order_yee.py)
 # --------------------------------------------------------------------------
 
-def update_B(E, B, p: Params, do_filter=False):
-    """update_B (first_order_yee.py:96-162), pml_state None."""
+# --------------------------------------------------------------------------
+# PML (boundary_conditions/PML.py)
+# --------------------------------------------------------------------------
+
+PML_WALLS = ("-x", "+x", "-y", "+y", "-z", "+z")
+
+
+def pml_layers(raw, p: Params):
+    """load_pml_from_toml's validation and sigma_max (PML.py:17-81): raw is a
+    dict or a list of dicts {wall, thickness, order=3, target_reflection=1e-8,
+    sigma_max (optional)}; returns [(wall, axis, thickness, order, sigma_max)]."""
+    raw = [] if raw is None else ([raw] if isinstance(raw, dict) else list(raw))
+    out, seen = [], set()
+    for r in raw:
+        wall = r.get("wall")
+        if wall not in PML_WALLS:
+            raise ValueError(f"Invalid PML wall: {wall}. Expected one of {list(PML_WALLS)}")
+        if wall in seen:
+            raise ValueError(f"Duplicate PML wall: {wall}")
+        seen.add(wall)
+        axis = "xyz".index(wall[1])
+        thickness = int(r.get("thickness", 0))
+        if thickness <= 0:
+            raise ValueError(f"PML thickness for {wall} must be positive")
+        if thickness > p.N[axis]:
+            raise ValueError(f"PML thickness for {wall} exceeds active cells on {wall[1]}: "
+                             f"{thickness} > {p.N[axis]}")
+        order = float(r.get("order", 3.0))
+        if "sigma_max" in r:
+            smax = float(r["sigma_max"])
+        else:
+            smax = -((order + 1.0) * p.C * math.log(float(r.get("target_reflection", 1.0e-8)))) / (
+                2.0 * thickness * p.d[axis])
+        out.append((wall, axis, thickness, order, smax))
+    return out
+
+
+def pml_profiles(layers, p: Params):
+    """build_pml (PML.py:84-122): (sigma_x, sigma_y, sigma_z) of shape N+2
+    per axis; layer cell i (0 at the interface) of thickness T holds
+    sigma_max ((i+1)/T)^order, counted from the interface outwards."""
+    shape = tuple(n + 2 for n in p.N)
+    prof = [np.zeros(shape) for _ in range(3)]
+    for wall, axis, T, order, smax in layers:
+        for i in range(T):
+            idx = T - i if wall[0] == "-" else shape[axis] - 1 - T + i
+            sl = [slice(None)] * 3
+            sl[axis] = idx
+            prof[axis][tuple(sl)] = smax * ((i + 1) / T) ** order
+    return prof
+
+
+def pml_ghosted(profile, p: Params):
+    """_tile_scalar_profile for one tile (PML.py:158-189): the one-ghost
+    profile placed at offset g-1 in a field-shaped array (zeros elsewhere)."""
+    g = p.g
+    out = np.zeros(p.shape)
+    n = profile.shape
+    out[g - 1:g - 1 + n[0], g - 1:g - 1 + n[1], g - 1:g - 1 + n[2]] = profile
+    return out
+
+
+def pml_stretch(deriv, memory, sigma, dt):
+    """stretch_spatial_derivative (PML.py:251-267): returns (deriv', memory')."""
+    b = np.exp(-sigma * dt)
+    m = b * memory + (b - 1.0) * deriv
+    return deriv + m, m
+
+
+# derivative order of the memory tuples and the axis of each derivative's
+# stretch (PML.py:270-359): (d?z/dy, d?y/dz, d?x/dz, d?z/dx, d?y/dx, d?x/dy)
+PML_DERIV_AXIS = (1, 2, 2, 0, 0, 1)
+
+
+def _pml_curl(derivs, memory, profiles, p: Params):
+    g = p.g
+    A = slice(g, -g)
+    sig = [pr[A, A, A] for pr in profiles]
+    new_d, new_m = [], []
+    for dv, m, ax in zip(derivs, memory, PML_DERIV_AXIS):
+        a, b = pml_stretch(dv, m, sig[ax], p.dt)
+        new_d.append(a)
+        new_m.append(b)
+    d = new_d
+    return (d[0] - d[1], d[2] - d[3], d[4] - d[5]), new_m
+
+
+def update_B(E, B, p: Params, do_filter=False, pml=None):
+    """update_B (first_order_yee.py:96-162).  pml: None or (b_memory (6
+    interior arrays), ghosted profiles (3)); with it returns (B, b_memory')."""
     g = p.g
     Ex, Ey, Ez = [refresh(c, p) for c in E]
     dt = p.dt / 2
@@ -669,7 +763,11 @@ def update_B(E, B, p: Params, do_filter=False):
     dEx_dy = (Ex[A, F, A] - Ex[A, A, A]) / dy
     dEz_dx = (Ez[F, A, A] - Ez[A, A, A]) / dx
     dEy_dx = (Ey[F, A, A] - Ey[A, A, A]) / dx
-    curl = (dEz_dy - dEy_dz, dEx_dz - dEz_dx, dEy_dx - dEx_dy)
+    derivs = (dEz_dy, dEy_dz, dEx_dz, dEz_dx, dEy_dx, dEx_dy)
+    if pml is None:
+        curl = (dEz_dy - dEy_dz, dEx_dz - dEz_dx, dEy_dx - dEx_dy)
+    else:
+        curl, mem = _pml_curl(derivs, pml[0], pml[1], p)
     out = []
     for c in range(3):
         b = np.array(B[c], dtype=np.float64, copy=True)
@@ -677,11 +775,13 @@ def update_B(E, B, p: Params, do_filter=False):
         out.append(b)
     if do_filter:
         out = [digital_filter(refresh(b, p), p.alpha, g) for b in out]
-    return [refresh(b, p) for b in out]
+    out = [refresh(b, p) for b in out]
+    return out if pml is None else (out, mem)
 
 
-def update_E(E, B, J, p: Params):
-    """update_E (first_order_yee.py:12-93), pml_state None."""
+def update_E(E, B, J, p: Params, pml=None):
+    """update_E (first_order_yee.py:12-93).  pml: None or (e_memory (6
+    interior arrays), ghosted profiles (3)); with it returns (E, e_memory')."""
     g = p.g
     Bx, By, Bz = [refresh(c, p) for c in B]
     dx, dy, dz = p.d
@@ -693,7 +793,10 @@ def update_E(E, B, J, p: Params):
     dBx_dy = (Bx[A, A, A] - Bx[A, K, A]) / dy
     dBz_dx = (Bz[A, A, A] - Bz[K, A, A]) / dx
     dBy_dx = (By[A, A, A] - By[K, A, A]) / dx
-    curl = (dBz_dy - dBy_dz, dBx_dz - dBz_dx, dBy_dx - dBx_dy)
+    if pml is None:
+        curl = (dBz_dy - dBy_dz, dBx_dz - dBz_dx, dBy_dx - dBx_dy)
+    else:
+        curl, mem = _pml_curl((dBz_dy, dBy_dz, dBx_dz, dBz_dx, dBy_dx, dBx_dy), pml[0], pml[1], p)
     out = []
     for c in range(3):
         e = np.array(E[c], dtype=np.float64, copy=True)
@@ -708,7 +811,8 @@ def update_E(E, B, J, p: Params):
         Ex, Ez = zero_boundary(Ex, p, 1), zero_boundary(Ez, p, 1)
     if p.bc[2] == BC_CONDUCTING:
         Ex, Ey = zero_boundary(Ex, p, 2), zero_boundary(Ey, p, 2)
-    return [refresh(e, p) for e in (Ex, Ey, Ez)]
+    out = [refresh(e, p) for e in (Ex, Ey, Ez)]
+    return out if pml is None else (out, mem)
 
 
 # --------------------------------------------------------------------------

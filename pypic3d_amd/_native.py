@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libpypic3d_hip.so")
 MAX_SPECIES = 16
 PUSHERS = {"boris": 0, "higuera_cary": 1}
 FUSED_VARIANTS = 6
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 c_int = ctypes.c_int
 c_i32 = ctypes.c_int32
@@ -50,7 +50,12 @@ class PicParticles(ctypes.Structure):
     _fields_ = [("x", c_vp * 3), ("u", c_vp * 3), ("active", c_vp)]
 
 
+class PicPml(ctypes.Structure):
+    _fields_ = [("mem", c_vp * 6), ("sigma", c_vp * 3)]
+
+
 DP = ctypes.POINTER(PicDesc)
+PMLP = ctypes.POINTER(PicPml)
 SP = ctypes.POINTER(PicSpecies)
 PP = ctypes.POINTER(PicParticles)
 
@@ -88,6 +93,9 @@ SIGNATURES = {
     "pic_cg_workspace_bytes": (c_i64, [DP]),
     "pic_poisson_cg": (c_int, [DP, c_vp, c_vp, c_dbl, c_dbl, c_i32, c_vp, c_i64,
                                ctypes.POINTER(c_i32), ctypes.POINTER(c_dbl), c_vp]),
+    "pic_yee_update_b_pml": (c_int, [DP, P3, P3, P3, PMLP, c_vp]),
+    "pic_yee_update_e_pml": (c_int, [DP, P3, P3, P3, P3, PMLP, c_vp]),
+    "pic_pml_stretch": (c_int, [DP, c_vp * 6, PMLP, c_vp]),
     "pic_efield_from_phi": (c_int, [DP, c_vp, P3, c_vp]),
     "pic_energy_workspace_bytes": (c_i64, [DP, SP]),
     "pic_energy": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, c_i64, c_vp]),

@@ -55,7 +55,7 @@ class EnsembleRunner:
             if static.current_deposition != "esirkepov" or static.electrostatic:
                 raise NotImplementedError("ensembles run the electrodynamic Esirkepov step")
             if fields[6] is not None:
-                raise NotImplementedError("PML is not implemented on the GPU path")
+                raise NotImplementedError("ensembles run without PML (pml_state must be None)")
             desc = make_desc(static, dyn)
             xp, up, act = particle_planes(parts)
             S, P = xp[0].shape

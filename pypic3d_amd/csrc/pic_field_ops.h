@@ -167,4 +167,67 @@ __device__ inline void yee_e_point(const PicDesc& d, const YeeConst& y, const fl
     out[2][c] = E[2][c] + (y.c2 * curl_z - J[2][c] * y.inv_eps) * y.dt;
 }
 
+// PML coordinate stretch of the six curl derivatives (PML.py:251-359): for a
+// derivative d along axis a at interior point (i, j, k), with b = exp(-sigma_a
+// dt) from the ghosted profile of axis a, the ADE memory becomes
+// m' = b m + (b - 1) d and the derivative d + m'.  Derivative order (and memory
+// order): d?z/dy, d?y/dz, d?x/dz, d?z/dx, d?y/dx, d?x/dy; curl = (d0 - d1,
+// d2 - d3, d4 - d5).  mem[] are interior (Nx, Ny, Nz) arrays updated in place.
+struct PmlArgs {
+    float* mem[6];
+    const float* sigma[3];
+    float dt;
+};
+
+__device__ inline void pml_stretch6(const PicDesc& d, const PmlArgs& pml, int i, int j, int k, float dv[6]) {
+    const int64_t c = fidx(d, i, j, k);
+    const int64_t mi = ((int64_t)(i - d.g) * d.n[1] + (j - d.g)) * d.n[2] + (k - d.g);
+    const float b[3] = {expf(-pml.sigma[0][c] * pml.dt), expf(-pml.sigma[1][c] * pml.dt),
+                        expf(-pml.sigma[2][c] * pml.dt)};
+    constexpr int kAxis[6] = {1, 2, 2, 0, 0, 1};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const float bb = b[kAxis[q]];
+        const float m = bb * pml.mem[q][mi] + (bb - 1.0f) * dv[q];
+        pml.mem[q][mi] = m;
+        dv[q] += m;
+    }
+}
+
+// yee_b_point with the PML stretch of the E derivatives (first_order_yee.py:127-142).
+__device__ inline void yee_b_point_pml(const PicDesc& d, const YeeConst& y, const PmlArgs& pml,
+                                       const float* const E[3], const float* const B[3], float* const out[3], int i,
+                                       int j, int k) {
+    int64_t c = fidx(d, i, j, k);
+    float ex = E[0][c], ey = E[1][c], ez = E[2][c];
+    float dv[6] = {(refreshed_at(E[2], d, y.mode, i, j + 1, k, 1) - ez) * y.id[1],
+                   (refreshed_at(E[1], d, y.mode, i, j, k + 1, 2) - ey) * y.id[2],
+                   (refreshed_at(E[0], d, y.mode, i, j, k + 1, 2) - ex) * y.id[2],
+                   (refreshed_at(E[2], d, y.mode, i + 1, j, k, 0) - ez) * y.id[0],
+                   (refreshed_at(E[1], d, y.mode, i + 1, j, k, 0) - ey) * y.id[0],
+                   (refreshed_at(E[0], d, y.mode, i, j + 1, k, 1) - ex) * y.id[1]};
+    pml_stretch6(d, pml, i, j, k, dv);
+    out[0][c] = B[0][c] - y.hdt * (dv[0] - dv[1]);
+    out[1][c] = B[1][c] - y.hdt * (dv[2] - dv[3]);
+    out[2][c] = B[2][c] - y.hdt * (dv[4] - dv[5]);
+}
+
+// yee_e_point with the PML stretch of the B derivatives (first_order_yee.py:49-72).
+__device__ inline void yee_e_point_pml(const PicDesc& d, const YeeConst& y, const PmlArgs& pml,
+                                       const float* const E[3], const float* const B[3], const float* const J[3],
+                                       float* const out[3], int i, int j, int k) {
+    int64_t c = fidx(d, i, j, k);
+    float bx = B[0][c], by = B[1][c], bz = B[2][c];
+    float dv[6] = {(bz - refreshed_at(B[2], d, y.mode, i, j - 1, k, 1)) * y.id[1],
+                   (by - refreshed_at(B[1], d, y.mode, i, j, k - 1, 2)) * y.id[2],
+                   (bx - refreshed_at(B[0], d, y.mode, i, j, k - 1, 2)) * y.id[2],
+                   (bz - refreshed_at(B[2], d, y.mode, i - 1, j, k, 0)) * y.id[0],
+                   (by - refreshed_at(B[1], d, y.mode, i - 1, j, k, 0)) * y.id[0],
+                   (bx - refreshed_at(B[0], d, y.mode, i, j - 1, k, 1)) * y.id[1]};
+    pml_stretch6(d, pml, i, j, k, dv);
+    out[0][c] = E[0][c] + (y.c2 * (dv[0] - dv[1]) - J[0][c] * y.inv_eps) * y.dt;
+    out[1][c] = E[1][c] + (y.c2 * (dv[2] - dv[3]) - J[1][c] * y.inv_eps) * y.dt;
+    out[2][c] = E[2][c] + (y.c2 * (dv[4] - dv[5]) - J[2][c] * y.inv_eps) * y.dt;
+}
+
 }  // namespace pic

@@ -113,6 +113,9 @@ struct F3 {
 struct W3f {
     float* p[3];
 };
+struct W6f {
+    float* p[6];
+};
 
 // first_order_yee.py:96-142: B_out = B_in - (dt/2) curl E (forward diffs).
 __global__ void yee_b_kernel(PicDesc d, YeeConst y, F3 E, F3 B, W3f out) {
@@ -126,6 +129,33 @@ __global__ void yee_e_kernel(PicDesc d, YeeConst y, F3 E, F3 B, F3 J, W3f out) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d.n[2]) return;
     yee_e_point(d, y, E.p, B.p, J.p, out.p, blockIdx.z + d.g, blockIdx.y + d.g, k + d.g);
+}
+
+__global__ void yee_b_pml_kernel(PicDesc d, YeeConst y, PmlArgs pml, F3 E, F3 B, W3f out) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.n[2]) return;
+    yee_b_point_pml(d, y, pml, E.p, B.p, out.p, blockIdx.z + d.g, blockIdx.y + d.g, k + d.g);
+}
+
+__global__ void yee_e_pml_kernel(PicDesc d, YeeConst y, PmlArgs pml, F3 E, F3 B, F3 J, W3f out) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.n[2]) return;
+    yee_e_point_pml(d, y, pml, E.p, B.p, J.p, out.p, blockIdx.z + d.g, blockIdx.y + d.g, k + d.g);
+}
+
+// stretch_spatial_derivative over six interior derivative arrays in place
+// (apply_tiled_pml_to_e_curl / _b_curl without the curl assembly).
+__global__ void pml_stretch_kernel(PicDesc d, PmlArgs pml, W6f dv) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.n[2]) return;
+    const int i = blockIdx.z + d.g, j = blockIdx.y + d.g;
+    const int64_t mi = ((int64_t)blockIdx.z * d.n[1] + blockIdx.y) * d.n[2] + k;
+    float v[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) v[q] = dv.p[q][mi];
+    pml_stretch6(d, pml, i, j, k + d.g, v);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) dv.p[q][mi] = v[q];
 }
 
 // electrostatic_yee.py:212-246: E interior = -(phi[+1]-phi[-1])/(2d), reading
@@ -286,6 +316,68 @@ int pic_yee_update_e(const PicDesc* desc, const float* const E_in[3], const floa
                        W3f{{E_out[0], E_out[1], E_out[2]}});
     PIC_CHECK_LAUNCH();
     return pic_refresh_fields(desc, E_out[0], E_out[1], E_out[2], 0, s);
+}
+
+static bool pml_ok(const PicPml* pml) {
+    if (!pml) return false;
+    for (int q = 0; q < 6; ++q)
+        if (!pml->mem[q]) return false;
+    for (int a = 0; a < 3; ++a)
+        if (!pml->sigma[a]) return false;
+    return true;
+}
+
+static PmlArgs pml_args(const PicDesc* desc, const PicPml* pml) {
+    PmlArgs a;
+    for (int q = 0; q < 6; ++q) a.mem[q] = pml->mem[q];
+    for (int c = 0; c < 3; ++c) a.sigma[c] = pml->sigma[c];
+    a.dt = (float)desc->dt;          // the stretch uses the full step in both updates (PML.py:291, 337)
+    return a;
+}
+
+int pic_yee_update_b_pml(const PicDesc* desc, const float* const E[3], const float* const B_in[3],
+                         float* const B_out[3], const PicPml* pml, void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && E && B_in && B_out && pml_ok(pml));
+    for (int c = 0; c < 3; ++c) PIC_REQUIRE(E[c] && B_in[c] && B_out[c]);
+    for (int c = 0; c < 3; ++c)
+        for (int e = 0; e < 3; ++e) PIC_REQUIRE((const float*)B_out[c] != E[e]);
+    hipStream_t s = as_stream(stream);
+    int block = interior_block(desc);
+    hipLaunchKernelGGL(yee_b_pml_kernel, interior_grid(desc, block), dim3(block), 0, s, *desc, yee_const(*desc),
+                       pml_args(desc, pml), F3{{E[0], E[1], E[2]}}, F3{{B_in[0], B_in[1], B_in[2]}},
+                       W3f{{B_out[0], B_out[1], B_out[2]}});
+    PIC_CHECK_LAUNCH();
+    return pic_refresh_fields(desc, B_out[0], B_out[1], B_out[2], 0, s);
+}
+
+int pic_yee_update_e_pml(const PicDesc* desc, const float* const E_in[3], const float* const B[3],
+                         const float* const J[3], float* const E_out[3], const PicPml* pml, void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && E_in && B && J && E_out && pml_ok(pml));
+    for (int c = 0; c < 3; ++c) PIC_REQUIRE(E_in[c] && B[c] && J[c] && E_out[c]);
+    for (int c = 0; c < 3; ++c)
+        for (int e = 0; e < 3; ++e) PIC_REQUIRE((const float*)E_out[c] != B[e] && (const float*)E_out[c] != J[e]);
+    hipStream_t s = as_stream(stream);
+    int block = interior_block(desc);
+    hipLaunchKernelGGL(yee_e_pml_kernel, interior_grid(desc, block), dim3(block), 0, s, *desc, yee_const(*desc),
+                       pml_args(desc, pml), F3{{E_in[0], E_in[1], E_in[2]}}, F3{{B[0], B[1], B[2]}},
+                       F3{{J[0], J[1], J[2]}}, W3f{{E_out[0], E_out[1], E_out[2]}});
+    PIC_CHECK_LAUNCH();
+    return pic_refresh_fields(desc, E_out[0], E_out[1], E_out[2], 0, s);
+}
+
+int pic_pml_stretch(const PicDesc* desc, float* const deriv[6], const PicPml* pml, void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && deriv && pml_ok(pml));
+    W6f dv;
+    for (int q = 0; q < 6; ++q) {
+        PIC_REQUIRE(deriv[q]);
+        dv.p[q] = deriv[q];
+    }
+    hipStream_t s = as_stream(stream);
+    int block = interior_block(desc);
+    hipLaunchKernelGGL(pml_stretch_kernel, interior_grid(desc, block), dim3(block), 0, s, *desc, pml_args(desc, pml),
+                       dv);
+    PIC_CHECK_LAUNCH();
+    return 0;
 }
 
 int pic_efield_from_phi(const PicDesc* desc, const float* phi, float* const E[3], void* stream) {

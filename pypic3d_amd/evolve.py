@@ -18,7 +18,7 @@ from .deposition.J_from_rhov import deposit_direct_into
 from .particles.particle_class import TiledParticles
 from .pusher.particle_push import check_pusher
 from .solvers.electrostatic_yee import electrostatic_fields_into
-from .solvers.first_order_yee import _check_pml, yee_b, yee_e
+from .solvers.first_order_yee import _check_pml, _pml_args, yee_b, yee_e
 from .utils import is_all_zero
 
 __all__ = ["time_loop_electrodynamic", "time_loop_electrostatic"]
@@ -86,12 +86,18 @@ def time_loop_electrodynamic(particles, species_config, fields, static_parameter
         N.call("pic_particle_boundaries", desc, sp, pic_particles(xf, u_planes, act_h), stream)
         new_particles = TiledParticles(x=xv, u=uv, active=act_h.view(1, 1, 1, S, P))
 
+    pml_b = pml_e = None
+    if pml_state is not None:      # PML memories advance in cloned arrays (first_order_yee.py:54-59, 133-138)
+        b_mem, prof3, b_new = _pml_args(desc, pml_state, 1)
+        e_mem, _, e_new = _pml_args(desc, pml_state, 0)
+        pml_b, pml_e = (b_mem, prof3), (e_mem, prof3)
+        pml_state = (e_new, b_new, pml_state[2])
     B1 = [b.new_empty(b.shape) for b in B3]
-    yee_b(desc, E3, B3, B1, False, stream)
+    yee_b(desc, E3, B3, B1, False, stream, pml=pml_b)
     E1 = [e.new_empty(e.shape) for e in E3]
-    yee_e(desc, E3, B1, J3, E1, stream)
+    yee_e(desc, E3, B1, J3, E1, stream, pml=pml_e)
     B2 = [b.new_empty(b.shape) for b in B3]
-    yee_b(desc, E1, B1, B2, True, stream)
+    yee_b(desc, E1, B1, B2, True, stream, pml=pml_b)
 
     out_fields = (
         tuple(like_input(e, t) for e, t in zip(E1, E)),

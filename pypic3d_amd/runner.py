@@ -18,7 +18,7 @@ from ._tensors import as_field3, check_fields, make_desc, make_species, particle
 from .deposition.Esirkepov import finish_current
 from .deposition.J_from_rhov import deposit_direct_into
 from .particles.particle_class import TiledParticles
-from .solvers.first_order_yee import yee_b, yee_e
+from .solvers.first_order_yee import _check_pml, _pml_args, yee_b, yee_e
 from .sorting import key_bits
 from .utils import is_all_zero
 
@@ -33,8 +33,7 @@ class ElectrodynamicRunner:
         the fused push + Esirkepov kernel (3-D first order) or the direct
         deposit of the J_from_rhov branch."""
         E, B, J, rho, phi, external_fields, pml_state, overflow = fields
-        if pml_state is not None:
-            raise NotImplementedError("PML is not implemented on the GPU path")
+        _check_pml(pml_state)
         if static_parameters.particle_pusher not in ("boris", "higuera_cary"):
             raise ValueError(f"Unknown particle_pusher: {static_parameters.particle_pusher}")
         self.static, self.dynamic = static_parameters, dynamic_parameters
@@ -66,6 +65,12 @@ class ElectrodynamicRunner:
         check_fields(self.desc, *self.E[0], *self.B, *self.J, *ext_E, *ext_B)
         self.Ee = None if all(is_all_zero(c) for c in ext_E) else N.p3(ext_E)
         self.Be = None if all(is_all_zero(c) for c in ext_B) else N.p3(ext_B)
+        self.pml_b = self.pml_e = None
+        if pml_state is not None:      # the runner advances its own copies of the PML memories in place
+            b_mem, prof3, b_new = _pml_args(self.desc, pml_state, 1)
+            e_mem, _, e_new = _pml_args(self.desc, pml_state, 0)
+            self.pml_b, self.pml_e = (b_mem, prof3), (e_mem, prof3)
+            pml_state = (e_new, b_new, pml_state[2])
         self.rest = (rho, phi, external_fields, pml_state, overflow)
         self.sort_every = int(sort_every)
         self.steps_done = 0
@@ -140,9 +145,9 @@ class ElectrodynamicRunner:
         if ev:
             ev[1].record()
         ev = self._mark("yee")
-        yee_b(d, E, self.B, self.Bt, False, s, scratch=self.scratch)
-        yee_e(d, E, self.Bt, self.J, Et, s, scratch=self.scratch)
-        yee_b(d, Et, self.Bt, self.B, True, s, scratch=self.scratch)
+        yee_b(d, E, self.B, self.Bt, False, s, scratch=self.scratch, pml=self.pml_b)
+        yee_e(d, E, self.Bt, self.J, Et, s, scratch=self.scratch, pml=self.pml_e)
+        yee_b(d, Et, self.Bt, self.B, True, s, scratch=self.scratch, pml=self.pml_b)
         if ev:
             ev[1].record()
         self.cur = dst

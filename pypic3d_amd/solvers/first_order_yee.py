@@ -4,24 +4,62 @@
 differences); ``update_E`` advances E by dt from C^2 curl B - J/eps (backward
 differences), then applies the digital filter and conducting walls.  The
 digital filter is the identity at alpha = 1 (centre weight 1, faces 0), so it
-is skipped there.  PML is not part of this library: a non-None ``pml_state``
-raises NotImplementedError.
+is skipped there.  With a ``pml_state`` = (e_memory, b_memory, profiles)
+(boundary_conditions/PML.py) the six curl derivatives are coordinate-stretched
+inside the kernels (first_order_yee.py:54-59, 133-138) and the updated
+memory is returned in a new state.
 """
 
 from __future__ import annotations
+
+import ctypes
+
+import torch
 
 from .. import _native as N
 from .._tensors import as_field3, check_fields, like_input, make_desc
 
 
 def _check_pml(pml_state):
-    if pml_state is not None:
-        raise NotImplementedError("PML curl stretching is not implemented on the GPU path (pml_state must be None)")
+    """pml_state must be None or the 3-tuple (e_memory[6], b_memory[6],
+    profiles[3]) of PML.initialize_(tiled_)pml_state."""
+    if pml_state is None:
+        return
+    if not (isinstance(pml_state, (tuple, list)) and len(pml_state) == 3 and len(pml_state[0]) == 6
+            and len(pml_state[1]) == 6 and len(pml_state[2]) == 3):
+        raise ValueError("pml_state must be (e_memory[6], b_memory[6], profiles[3])")
 
 
-def yee_b(desc, E3, B3, out3, do_filter, stream, scratch=None):
-    """B half step on 3-D tensors into out3 (out3 must not alias E3)."""
-    N.call("pic_yee_update_b", desc, N.p3(E3), N.p3(B3), N.p3(out3), stream)
+def pml_memory3(desc, mem):
+    """The six memory arrays as contiguous (Nx, Ny, Nz) float32 tensors."""
+    out = []
+    n = tuple(int(v) for v in desc.n)
+    for m in mem:
+        t = m.reshape(n)
+        if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError("PML memory must be contiguous float32 CUDA tensors of the interior shape")
+        out.append(t)
+    return out
+
+
+def pml_struct(mem6, sigma3):
+    """PicPml over six (Nx, Ny, Nz) memory tensors and three field-shaped profiles."""
+    s = N.PicPml()
+    for q in range(6):
+        s.mem[q] = mem6[q].data_ptr()
+    for a in range(3):
+        s.sigma[a] = sigma3[a].data_ptr()
+    return s
+
+
+def yee_b(desc, E3, B3, out3, do_filter, stream, scratch=None, pml=None):
+    """B half step on 3-D tensors into out3 (out3 must not alias E3).  pml:
+    None or (memory tensors[6], profile tensors[3]), memory updated in place."""
+    if pml is None:
+        N.call("pic_yee_update_b", desc, N.p3(E3), N.p3(B3), N.p3(out3), stream)
+    else:
+        N.call("pic_yee_update_b_pml", desc, N.p3(E3), N.p3(B3), N.p3(out3), ctypes.byref(pml_struct(*pml)),
+               stream)
     if do_filter and desc.alpha != 1.0:
         tmp = scratch if scratch is not None else [c.clone() for c in out3]
         for c, t in zip(out3, tmp):
@@ -31,9 +69,14 @@ def yee_b(desc, E3, B3, out3, do_filter, stream, scratch=None):
     return out3
 
 
-def yee_e(desc, E3, B3, J3, out3, stream, scratch=None):
-    """E full step on 3-D tensors into out3 (out3 must not alias B3 or J3)."""
-    N.call("pic_yee_update_e", desc, N.p3(E3), N.p3(B3), N.p3(J3), N.p3(out3), stream)
+def yee_e(desc, E3, B3, J3, out3, stream, scratch=None, pml=None):
+    """E full step on 3-D tensors into out3 (out3 must not alias B3 or J3).
+    pml as for yee_b."""
+    if pml is None:
+        N.call("pic_yee_update_e", desc, N.p3(E3), N.p3(B3), N.p3(J3), N.p3(out3), stream)
+    else:
+        N.call("pic_yee_update_e_pml", desc, N.p3(E3), N.p3(B3), N.p3(J3), N.p3(out3),
+               ctypes.byref(pml_struct(*pml)), stream)
     if desc.alpha != 1.0:
         tmp = scratch if scratch is not None else [c.clone() for c in out3]
         for c, t in zip(out3, tmp):
@@ -49,6 +92,17 @@ def yee_e(desc, E3, B3, J3, out3, stream, scratch=None):
     return out3
 
 
+def _pml_args(desc, pml_state, which):
+    """(cloned memory[6] as (Nx, Ny, Nz) tensors, profiles[3] as field tensors,
+    the cloned memories in the caller's shapes) for which = 0 (E) or 1 (B)."""
+    mem_in = pml_state[which]
+    clones = [m.clone() for m in mem_in]
+    mem3 = pml_memory3(desc, clones)
+    prof3 = [as_field3(pr, "PML profile") for pr in pml_state[2]]
+    check_fields(desc, *prof3)
+    return mem3, prof3, tuple(clones)
+
+
 def update_B(E_tiles, B_tiles, static_parameters, dynamic_parameters, pml_state=None, do_filter=False):
     """Returns (B, pml_state) (first_order_yee.py:96-162)."""
     _check_pml(pml_state)
@@ -57,7 +111,12 @@ def update_B(E_tiles, B_tiles, static_parameters, dynamic_parameters, pml_state=
     B3 = [as_field3(c, "B") for c in B_tiles]
     check_fields(desc, *E3, *B3)
     out = [b.new_empty(b.shape) for b in B3]
-    yee_b(desc, E3, B3, out, do_filter, N.stream_handle())
+    pml = None
+    if pml_state is not None:
+        mem3, prof3, clones = _pml_args(desc, pml_state, 1)
+        pml = (mem3, prof3)
+        pml_state = (pml_state[0], clones, pml_state[2])
+    yee_b(desc, E3, B3, out, do_filter, N.stream_handle(), pml=pml)
     return tuple(like_input(o, t) for o, t in zip(out, B_tiles)), pml_state
 
 
@@ -70,5 +129,10 @@ def update_E(E_tiles, B_tiles, J_tiles, static_parameters, dynamic_parameters, p
     J3 = [as_field3(c, "J") for c in J_tiles]
     check_fields(desc, *E3, *B3, *J3)
     out = [e.new_empty(e.shape) for e in E3]
-    yee_e(desc, E3, B3, J3, out, N.stream_handle())
+    pml = None
+    if pml_state is not None:
+        mem3, prof3, clones = _pml_args(desc, pml_state, 0)
+        pml = (mem3, prof3)
+        pml_state = (clones, pml_state[1], pml_state[2])
+    yee_e(desc, E3, B3, J3, out, N.stream_handle(), pml=pml)
     return tuple(like_input(o, t) for o, t in zip(out, E_tiles)), pml_state
