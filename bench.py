@@ -120,6 +120,53 @@ def extras(n, ppc, steps, seed):
                                  "hbm_gbs_at_40B_per_cell_iter": 40 * n_cells * k / (ms_cg * 1e-3) / 1e9}
         del r
         torch.cuda.empty_cache()
+    out["yee_pml"] = yee_pml(n)
+    return out
+
+
+def yee_pml(n, layers=16, reps=20):
+    """The three Yee updates of one step on the n^3 grid without and with a PML
+    of `layers` cells on all six walls (algorithmic bytes: 120 B/cell without;
+    with it each of the three updates also reads its six ADE memories and the
+    three profiles, 36 B/cell, and writes a memory where its axis' profile is
+    nonzero)."""
+    import torch
+    from pypic3d_amd import _native as N
+    from pypic3d_amd import workloads
+    from pypic3d_amd._tensors import make_desc
+    from pypic3d_amd.boundary_conditions.PML import initialize_tiled_pml_state, load_pml_from_toml
+    from pypic3d_amd.solvers.first_order_yee import _pml_args, yee_b, yee_e
+
+    static, dyn = workloads.make_parameters((n, n, n), cfl=0.5)
+    desc = make_desc(static, dyn)
+    f = workloads.empty_fields(static, dyn)
+    E = [c[0, 0, 0] for c in f[0]]
+    B = [c[0, 0, 0] for c in f[1]]
+    J = [c[0, 0, 0] for c in f[2]]
+    E2 = [c.clone() for c in E]
+    Bt = [c.clone() for c in B]
+    walls = [{"wall": w, "thickness": layers} for w in ("-x", "+x", "-y", "+y", "-z", "+z")]
+    _, _, _, _, prof = load_pml_from_toml(walls, static, dyn)
+    state = initialize_tiled_pml_state(static, dyn, prof)
+    pb, pe = _pml_args(desc, state, 1), _pml_args(desc, state, 0)
+    s = N.stream_handle()
+    out = {}
+    for name, pml_b, pml_e in (("none", None, None), ("six_walls", pb[:2], pe[:2])):
+        def step():
+            yee_b(desc, E, B, Bt, False, s, pml=pml_b)
+            yee_e(desc, E, Bt, J, E2, s, pml=pml_e)
+            yee_b(desc, E2, Bt, B, True, s, pml=pml_b)
+        ms = _timed(step, reps)
+        cells = n ** 3
+        if name == "none":
+            bytes_ = 120 * cells
+        else:
+            g = desc.g
+            inside = [int((t[0, 0, 0, g:-g, g:-g, g:-g] != 0).sum()) for t in state[2]]
+            writes = 4 * sum(inside[a] for a in (1, 2, 2, 0, 0, 1))
+            bytes_ = 120 * cells + 3 * (36 * cells + writes)
+        out[name] = {"ms_per_step": ms, "hbm_gbs": bytes_ / (ms * 1e-3) / 1e9}
+    out["layers"] = layers
     return out
 
 

@@ -189,7 +189,7 @@ __device__ inline void pml_stretch6(const PicDesc& d, const PmlArgs& pml, int i,
     for (int q = 0; q < 6; ++q) {
         const float bb = b[kAxis[q]];
         const float m = bb * pml.mem[q][mi] + (bb - 1.0f) * dv[q];
-        pml.mem[q][mi] = m;
+        if (bb != 1.0f) pml.mem[q][mi] = m;     // sigma = 0: m' = m, nothing to write
         dv[q] += m;
     }
 }
