@@ -9,7 +9,8 @@
 // differences D = S(new) - S(old) are formed in FP64 so the ~v*dt/dx sized
 // differences do not lose digits to cancellation.  The 3-D weight products and
 // all accumulation are FP32.
-#include "pic_particle_math.h"
+#include "pic_exact_grid.h"
+#include "pic_fused_block.h"
 
 namespace {
 
@@ -21,6 +22,10 @@ constexpr int kBlock = 256;
 // kernels
 // ------------------------------------------------------------------------
 
+// kCic: first-order shape, two-point stencils per active axis (the weight of
+// point a-1 is exactly zero, shapes.py:46-48) and index g on reduced axes,
+// 8 loads per component; otherwise the general 3^3 gather.
+template <bool kCic>
 __global__ void push_kernel(PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, float* ux_o, float* uy_o,
                             float* uz_o, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -29,10 +34,34 @@ __global__ void push_kernel(PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs
     float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
     float nv[3] = {v[0], v[1], v[2]};
     if (in.active[i]) {
-        double pos[3];
-        load_pos(in, i, pos);
         float E[3], B[3];
-        gather_EB(d, fp, pos, E, B);
+        if constexpr (kCic) {
+            CicAxis gc[3], gv[3];
+            for (int a = 0; a < 3; ++a) {
+                if (d.n[a] > 1) {
+                    const float x = in.x[a][i];
+                    int an;
+                    float dl;
+                    cic_axis(x, (float)d.org_c[a], (float)(1.0 / d.sp_c[a]), d.org_c[a], d.sp_c[a], d.dims[a], gc[a],
+                             an, dl);
+                    cic_axis(x, (float)d.org_v[a], (float)(1.0 / d.sp_v[a]), d.org_v[a], d.sp_v[a], d.dims[a], gv[a],
+                             an, dl);
+                } else {
+                    reduced_cic(d.g, gc[a]);
+                    reduced_cic(d.g, gv[a]);
+                }
+            }
+            E[0] = gather_cic(fp.E[0], fp.Ee[0], d, gv[0], gc[1], gc[2]);
+            E[1] = gather_cic(fp.E[1], fp.Ee[1], d, gc[0], gv[1], gc[2]);
+            E[2] = gather_cic(fp.E[2], fp.Ee[2], d, gc[0], gc[1], gv[2]);
+            B[0] = gather_cic(fp.B[0], fp.Be[0], d, gc[0], gv[1], gv[2]);
+            B[1] = gather_cic(fp.B[1], fp.Be[1], d, gv[0], gc[1], gv[2]);
+            B[2] = gather_cic(fp.B[2], fp.Be[2], d, gv[0], gv[1], gc[2]);
+        } else {
+            double pos[3];
+            load_pos(in, i, pos);
+            gather_EB(d, fp, pos, E, B);
+        }
         float h = (float)(sp.charge[s] * d.dt / (2.0 * sp.mass[s]));
         push_velocity(nv, E, B, h, d);
         for (int a = 0; a < 3; ++a)
@@ -220,6 +249,123 @@ __global__ __launch_bounds__(kBlock) void tiled_deposit_kernel(PicDesc d, PicSpe
     }
 }
 
+// Tile-ordered rho / J_from_rhov deposit for 3-D first-order runs on an exact
+// grid whose spacing equals the cell size.  Thread t of a chunk owns four
+// consecutive sorted slots; a CIC particle deposits on the 2x2x2 nodes
+// (a..a+1) of its centre anchor a (node weights 1 - dn/d, dn/d; J face weights
+// 1 - df/d, df/d with df = dn - d/2, J_from_rhov.py:110-198, rho.py:72-150),
+// so particles of one cell share all 8*kC targets: their values are summed in
+// registers while the anchor stays and added into the LDS box when it
+// changes.  Anchors come from exact_anchors (FP32, equal to the FP64
+// evaluation); particles outside the exact range or the box take the general
+// per-particle deposit.
+template <int kC, int kT, bool kRho>
+__global__ __launch_bounds__(kBlock) void tiled_deposit_group_kernel(PicDesc d, PicSpecies sp, PicParticles in,
+                                                                     float* J0, float* J1, float* J2,
+                                                                     const int32_t* __restrict__ starts, int tiles,
+                                                                     int nt1, int nt2, ExactGrid eg) {
+    constexpr int kB = kT + 5;
+    constexpr int kV = kB * kB * kB;
+    constexpr int kP = 4;
+    constexpr int kN = 8 * kC;
+    __shared__ float box[kC * kV];
+    const int b = blockIdx.x;
+    const int64_t begin = starts[b], end = starts[b + 1];
+    if (begin >= end) return;
+    const int t = b % (tiles + 1);
+    const int s = b / (tiles + 1);
+    int o[3] = {0, 0, 0};
+    if (t < tiles) {
+        o[0] = (t / (nt1 * nt2)) * kT + d.g - 2;
+        o[1] = ((t / nt2) % nt1) * kT + d.g - 2;
+        o[2] = (t % nt2) * kT + d.g - 2;
+    }
+    for (int e = threadIdx.x; e < kC * kV; e += kBlock) box[e] = 0.0f;
+    __syncthreads();
+    const DepositBox<kC, kB> sink{{J0, J1, J2}, box, {o[0], o[1], o[2]}};
+    const float q = charge_density(d, sp, s);
+    auto flush = [&](int h0, int h1, int h2, const float* acc) {
+        float* p = box + ((h0 - o[0]) * kB + (h1 - o[1])) * kB + (h2 - o[2]);
+#pragma unroll
+        for (int m = 0; m < kN; ++m) {
+            const int c = m >> 3, n = m & 7;
+            atomicAdd(p + c * kV + ((n >> 2) * kB + ((n >> 1) & 1)) * kB + (n & 1), acc[m]);
+        }
+    };
+    for (int64_t base = begin & ~(int64_t)(kP - 1); base < end; base += kBlock * kP) {
+        const int64_t s0 = base + (int64_t)kP * threadIdx.x;
+        float acc[kN];
+#pragma unroll
+        for (int m = 0; m < kN; ++m) acc[m] = 0.0f;
+        int h0 = 0, h1 = 0, h2 = 0;
+        bool have = false;
+#pragma unroll 1
+        for (int j = 0; j < kP; ++j) {
+            const int64_t i = s0 + j;
+            if (i < begin || i >= end || !in.active[i]) continue;
+            float x[3];
+            int an[3];
+            float dn[3];
+            bool fast = true;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                x[a] = in.x[a][i];
+                int av;
+                float nd, dv;
+                fast = fast && x[a] >= eg.xlo[a] && x[a] < eg.xhi[a];
+                exact_anchors(x[a], eg.org_c[a], eg.sp_c[a], eg.isp_c[a], eg.half_sp[a], an[a], dn[a], nd, av, dv);
+                fast = fast && (unsigned)(an[a] - o[a]) <= (unsigned)(kB - 2);
+            }
+            if (!fast) {
+                if constexpr (kRho) rho_particle(d, q, in, i, sink);
+                else direct_particle(d, q, in, i, sink);
+                continue;
+            }
+            float wn[3][2], wf[3][2];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float r = dn[a] * eg.inv_d[a];
+                wn[a][0] = 1.0f - r;
+                wn[a][1] = r;
+                const float rf = (dn[a] - eg.half_sp[a]) * eg.inv_d[a];
+                wf[a][0] = 1.0f - rf;
+                wf[a][1] = rf;
+            }
+            float val[kN];
+            if constexpr (kRho) {
+#pragma unroll
+                for (int n = 0; n < 8; ++n) val[n] = q * wn[0][n >> 2] * wn[1][(n >> 1) & 1] * wn[2][n & 1];
+            } else {
+                const float qv[3] = {q * in.u[0][i], q * in.u[1][i], q * in.u[2][i]};
+#pragma unroll
+                for (int n = 0; n < 8; ++n) {
+                    const int xi = n >> 2, yi = (n >> 1) & 1, zi = n & 1;
+                    val[n] = qv[0] * wf[0][xi] * wn[1][yi] * wn[2][zi];
+                    val[8 + n] = qv[1] * wn[0][xi] * wf[1][yi] * wn[2][zi];
+                    val[16 + n] = qv[2] * wn[0][xi] * wn[1][yi] * wf[2][zi];
+                }
+            }
+            const bool same = have && an[0] == h0 && an[1] == h1 && an[2] == h2;
+            if (have && !same) flush(h0, h1, h2, acc);
+#pragma unroll
+            for (int m = 0; m < kN; ++m) acc[m] = same ? acc[m] + val[m] : val[m];
+            h0 = an[0];
+            h1 = an[1];
+            h2 = an[2];
+            have = true;
+        }
+        if (have) flush(h0, h1, h2, acc);
+    }
+    __syncthreads();
+    float* const J[3] = {J0, J1, J2};
+    for (int e = threadIdx.x; e < kC * kV; e += kBlock) {
+        const float v = box[e];
+        if (v == 0.0f) continue;
+        const int c = e / kV, r = e % kV;
+        deposit_at(J[c], d, o[0] + r / (kB * kB), o[1] + (r / kB) % kB, o[2] + r % kB, v);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -231,8 +377,12 @@ int pic_push(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, 
     for (int a = 0; a < 3; ++a) PIC_REQUIRE(E[a] && B[a] && u_out[a]);
     int64_t n = (int64_t)sp->count * sp->slots;
     if (n == 0) return 0;
-    hipLaunchKernelGGL(push_kernel, dim3(grid_1d(n, kBlock)), dim3(kBlock), 0, as_stream(stream), *desc, *sp, *in,
-                       make_fields(E, B, E_ext, B_ext), u_out[0], u_out[1], u_out[2], n);
+    if (desc->shape_factor == 1)
+        hipLaunchKernelGGL(push_kernel<true>, dim3(grid_1d(n, kBlock)), dim3(kBlock), 0, as_stream(stream), *desc, *sp,
+                           *in, make_fields(E, B, E_ext, B_ext), u_out[0], u_out[1], u_out[2], n);
+    else
+        hipLaunchKernelGGL(push_kernel<false>, dim3(grid_1d(n, kBlock)), dim3(kBlock), 0, as_stream(stream), *desc,
+                           *sp, *in, make_fields(E, B, E_ext, B_ext), u_out[0], u_out[1], u_out[2], n);
     PIC_CHECK_LAUNCH();
     return 0;
 }
@@ -296,13 +446,29 @@ static int tiled_deposit(const PicDesc* desc, const PicSpecies* sp, const PicPar
 #define PIC_TD(C, T, R)                                                                                           \
     hipLaunchKernelGGL((tiled_deposit_kernel<C, T, R>), dim3((unsigned)blocks), dim3(kBlock), 0, st, *desc, *sp, *in, \
                        J[0], j1, j2, starts, (int)tiles, nt[1], nt[2])
-    if (rho) {
+    ExactGrid eg;
+    bool grouped = desc->variant != 1 && desc->shape_factor == 1 && desc->n[0] > 1 && desc->n[1] > 1 &&
+                   desc->n[2] > 1 && make_exact_grid(*desc, eg);
+    for (int a = 0; a < 3 && grouped; ++a) grouped = desc->d[a] == desc->sp_c[a];
+#define PIC_TDG(C, T, R)                                                                                            \
+    hipLaunchKernelGGL((tiled_deposit_group_kernel<C, T, R>), dim3((unsigned)blocks), dim3(kBlock), 0, st, *desc, \
+                       *sp, *in, J[0], j1, j2, starts, (int)tiles, nt[1], nt[2], eg)
+    if (grouped) {
+        if (rho) {
+            if (tile == 8) PIC_TDG(1, 8, true);
+            else PIC_TDG(1, 4, true);
+        } else {
+            if (tile == 8) PIC_TDG(3, 8, false);
+            else PIC_TDG(3, 4, false);
+        }
+    } else if (rho) {
         if (tile == 8) PIC_TD(1, 8, true);
         else PIC_TD(1, 4, true);
     } else {
         if (tile == 8) PIC_TD(3, 8, false);
         else PIC_TD(3, 4, false);
     }
+#undef PIC_TDG
 #undef PIC_TD
     PIC_CHECK_LAUNCH();
     return 0;
