@@ -188,6 +188,21 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
                                    const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,
                                    const int32_t* starts, int32_t rounds, int32_t min_blocks, int32_t* workspace,
                                    void* stream);
+/* Fused particle phase of time_loop_electrodynamic's direct branch
+ * (evolve.py:44-67) for tile-sorted particles on an exact 3-D first-order grid
+ * whose cell size equals the node spacing (desc->d == desc->sp_c): gather +
+ * push, x += v dt/2, particle boundaries, J_from_rhov deposit into J (zeroed
+ * by the caller; fold, refresh and filter stay with the caller), x += v dt/2,
+ * particle boundaries.  Writes out (must not alias in); same results as
+ * pic_push, pic_advance_positions, pic_particle_boundaries,
+ * pic_deposit_direct, pic_advance_positions, pic_particle_boundaries up to
+ * FP32 rounding (an ulp in a velocity where the compiler fuses the push's
+ * multiply-adds differently) and summation order in J.  tile 4 or 8, min_blocks 1 or 4; workspace as
+ * for pic_fused_push_esirkepov_tiled. */
+int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                                const float* const E[3], const float* const B[3], const float* const E_ext[3],
+                                const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,
+                                const int32_t* starts, int32_t min_blocks, int32_t* workspace, void* stream);
 
 /* ---- cell sort (particle_initialization.py:89-102, 140-160) ------------- */
 /* key = s*(N+1) + (ix*Ny+iy)*Nz+iz with ix = clip(floor((x+W/2)/dx),0,Nx-1);

@@ -82,6 +82,8 @@ SIGNATURES = {
     "pic_fused_push_esirkepov": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, PP, P3, c_vp]),
     "pic_fused_push_esirkepov_tiled": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, PP, P3, c_i32, c_vp, c_i32, c_i32,
                                                c_vp, c_vp]),
+    "pic_fused_push_direct_tiled": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, PP, P3, c_i32, c_vp, c_i32, c_vp,
+                                            c_vp]),
     "pic_tile_exact_grid": (c_int, [DP]),
     "pic_cell_keys": (c_int, [DP, SP, PP, c_vp, c_vp]),
     "pic_tile_count": (c_i64, [DP, c_i32]),

@@ -605,6 +605,297 @@ __global__ __launch_bounds__(kThreads) void fused_tile_rare_kernel(PicDesc d, Pi
 
 __global__ void zero_count_kernel(int32_t* c) { *c = 0; }
 
+// Direct-current branch of time_loop_electrodynamic (evolve.py:44-67) on
+// tile-sorted particles of an exact 3-D first-order grid, fused: gather from
+// the tile's LDS field box, push, x += v dt/2 (FP64, rounded to FP32 like
+// pic_advance_positions), particle boundaries, the J_from_rhov deposit at the
+// half position (J_from_rhov.py:88-200: node weights 1 - dn/d, dn/d and face
+// weights 1 - df/d, df/d with df = dn - d/2 on the 2x2x2 nodes of the centre
+// anchor, summed in registers over a thread's equal-anchor particles and
+// added into an LDS J box), x += v dt/2, boundaries, store.  Particles the
+// fast loop does not cover are listed for fused_direct_rare_kernel.
+template <int kT, int kMinBlocks>
+__global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel(
+    PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, PicParticles out, float* Jx, float* Jy, float* Jz,
+    const int32_t* __restrict__ starts, TileGeom tg, ExactGrid eg, int32_t* __restrict__ rare, int vec_ok) {
+    constexpr int kP = 4;
+    constexpr int kB = kT + 5;
+    constexpr int kB3 = kB * kB * kB;
+    constexpr int kFB = kT + 4;
+    constexpr int kFV = kFB * kFB * kFB;
+    constexpr int kChunk = kThreads * kP;
+    __shared__ float box[3 * kB3];
+    __shared__ float fbox[6 * kFV];
+    const int b = blockIdx.x;
+    const int64_t begin = starts[b], end = starts[b + 1];
+    if (begin >= end) return;
+    const int t = b % (tg.tiles + 1);
+    const int s = b / (tg.tiles + 1);
+    int o[3] = {0, 0, 0}, fo[3] = {0, 0, 0};
+    if (t < tg.tiles) {
+        const int tx = t / (tg.nt[1] * tg.nt[2]), ty = (t / tg.nt[2]) % tg.nt[1], tz = t % tg.nt[2];
+        const int c0[3] = {tx * kT, ty * kT, tz * kT};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) o[a] = fo[a] = c0[a] + d.g - 2;
+    }
+    for (int e = threadIdx.x; e < 3 * kB3; e += kThreads) box[e] = 0.0f;
+    for (int e = threadIdx.x; e < 6 * kFV; e += kThreads) {
+        const int c = e / kFV, r = e % kFV;
+        const int i = pmod(fo[0] + r / (kFB * kFB), d.dims[0]), j = pmod(fo[1] + (r / kFB) % kFB, d.dims[1]);
+        const int k = pmod(fo[2] + r % kFB, d.dims[2]);
+        const int64_t id = fidx(d, i, j, k);
+        const float* F = c < 3 ? fp.E[c] : fp.B[c - 3];
+        const float* Fe = c < 3 ? fp.Ee[c] : fp.Be[c - 3];
+        float v = F[id];
+        if (Fe) v += Fe[id];
+        fbox[e] = v;
+    }
+    __syncthreads();
+    const FieldBox fb{fbox, {fo[0], fo[1], fo[2]}, kFB};
+    const SpeciesK k = species_constants(d, sp, s);
+    const double hdt = 0.5 * d.dt;
+    auto flush = [&](int h0, int h1, int h2, const float* acc) {
+        float* p = box + ((h0 - o[0]) * kB + (h1 - o[1])) * kB + (h2 - o[2]);
+#pragma unroll
+        for (int m = 0; m < 24; ++m) {
+            const int c = m >> 3, n = m & 7;
+            atomicAdd(p + c * kB3 + ((n >> 2) * kB + ((n >> 1) & 1)) * kB + (n & 1), acc[m]);
+        }
+    };
+
+    for (int64_t base = begin & ~(int64_t)(kP - 1); base < end; base += kChunk) {
+        const int64_t s0 = base + (int64_t)kP * threadIdx.x;
+        float acc[24];
+#pragma unroll
+        for (int m = 0; m < 24; ++m) acc[m] = 0.0f;
+        int h0 = 0, h1 = 0, h2 = 0;
+        bool have = false;
+        float OX[3][kP], OU[3][kP];
+        uint32_t oact = 0u;
+#pragma unroll 1
+        for (int j = 0; j < kP; ++j) {
+            const int64_t i = s0 + j;
+            float ox[3] = {0.0f, 0.0f, 0.0f}, ou[3] = {0.0f, 0.0f, 0.0f};
+            uint32_t oa = 0u;
+            do {
+            if (i < begin || i >= end) break;
+            if (!in.active[i]) break;
+            const float xf[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
+            float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
+            int anc[3], anv[3];
+            float dc[3], dv[3], nodec[3];
+            bool fast = true;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                fast = fast && xf[a] >= eg.xlo[a] && xf[a] < eg.xhi[a];
+                exact_anchors(xf[a], eg.org_c[a], eg.sp_c[a], eg.isp_c[a], eg.half_sp[a], anc[a], dc[a], nodec[a],
+                              anv[a], dv[a]);
+                fast = fast && (unsigned)(anc[a] - fo[a]) < (unsigned)(kFB - 1) &&
+                       (unsigned)(anv[a] - fo[a]) < (unsigned)(kFB - 1);
+            }
+            if (!fast) {
+                rare_push(rare, i);
+                break;
+            }
+            CicAxis gc[3], gv[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float rc = dc[a] * eg.isp_c[a], rv = dv[a] * eg.isp_c[a];
+                gc[a].p0 = gc[a].p1 = gv[a].p0 = gv[a].p1 = 0;
+                gc[a].w0 = 1.0f - rc;
+                gc[a].w1 = rc;
+                gv[a].w0 = 1.0f - rv;
+                gv[a].w1 = rv;
+            }
+            float E[3], Bf[3];
+            E[0] = gather_box(fb, 0, anv[0], anc[1], anc[2], gv[0], gc[1], gc[2]);
+            E[1] = gather_box(fb, 1, anc[0], anv[1], anc[2], gc[0], gv[1], gc[2]);
+            E[2] = gather_box(fb, 2, anc[0], anc[1], anv[2], gc[0], gc[1], gv[2]);
+            Bf[0] = gather_box(fb, 3, anc[0], anv[1], anv[2], gc[0], gv[1], gv[2]);
+            Bf[1] = gather_box(fb, 4, anv[0], anc[1], anv[2], gv[0], gc[1], gv[2]);
+            Bf[2] = gather_box(fb, 5, anv[0], anv[1], anc[2], gv[0], gv[1], gc[2]);
+            float nv[3] = {v[0], v[1], v[2]};
+            push_velocity(nv, E, Bf, k.h, d);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                if ((k.uu >> a) & 1u) v[a] = nv[a];
+            // ---- x += v dt/2, boundaries (evolve.py:48-52)
+            bool act = true;
+            float xh[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                double t = ((k.ux >> a) & 1u) ? (double)(float)step_pos((double)xf[a], (double)v[a], hdt)
+                                              : (double)xf[a];
+                if (d.pbc[a] == 0) periodic_wrap(d.wind[a], t);
+                else apply_axis_boundary(d.pbc[a], d.wind[a], t, v[a], act);
+                xh[a] = (float)t;
+            }
+            if (act) {
+                // ---- J_from_rhov at the half position (anchors exact in FP32)
+                int an[3];
+                float dn[3];
+                bool inbox = true;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    int av2;
+                    float nd, dv2;
+                    inbox = inbox && xh[a] >= eg.xlo[a] && xh[a] < eg.xhi[a];
+                    exact_anchors(xh[a], eg.org_c[a], eg.sp_c[a], eg.isp_c[a], eg.half_sp[a], an[a], dn[a], nd, av2,
+                                  dv2);
+                    inbox = inbox && (unsigned)(an[a] - o[a]) <= (unsigned)(kB - 2);
+                }
+                if (!inbox) {
+                    rare_push(rare, i);
+                    break;
+                }
+                float wn[3][2], wf[3][2];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float r = dn[a] * eg.inv_d[a];
+                    wn[a][0] = 1.0f - r;
+                    wn[a][1] = r;
+                    const float rf = (dn[a] - eg.half_sp[a]) * eg.inv_d[a];
+                    wf[a][0] = 1.0f - rf;
+                    wf[a][1] = rf;
+                }
+                const float qv[3] = {k.q_dv * v[0], k.q_dv * v[1], k.q_dv * v[2]};
+                float val[24];
+#pragma unroll
+                for (int n = 0; n < 8; ++n) {
+                    const int xi = n >> 2, yi = (n >> 1) & 1, zi = n & 1;
+                    val[n] = qv[0] * wf[0][xi] * wn[1][yi] * wn[2][zi];
+                    val[8 + n] = qv[1] * wn[0][xi] * wf[1][yi] * wn[2][zi];
+                    val[16 + n] = qv[2] * wn[0][xi] * wn[1][yi] * wf[2][zi];
+                }
+                const bool same = have && an[0] == h0 && an[1] == h1 && an[2] == h2;
+                if (have && !same) flush(h0, h1, h2, acc);
+#pragma unroll
+                for (int m = 0; m < 24; ++m) acc[m] = same ? acc[m] + val[m] : val[m];
+                h0 = an[0];
+                h1 = an[1];
+                h2 = an[2];
+                have = true;
+                // ---- x += v dt/2, boundaries (evolve.py:62-66)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    double t2 = ((k.ux >> a) & 1u) ? (double)(float)step_pos((double)xh[a], (double)v[a], hdt)
+                                                   : (double)xh[a];
+                    if (d.pbc[a] == 0) periodic_wrap(d.wind[a], t2);
+                    else apply_axis_boundary(d.pbc[a], d.wind[a], t2, v[a], act);
+                    ox[a] = (float)t2;
+                    ou[a] = v[a];
+                }
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    ox[a] = act ? ox[a] : 0.0f;
+                    ou[a] = act ? ou[a] : 0.0f;
+                }
+                oa = act ? 1u : 0u;
+            }
+            } while (false);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+#pragma unroll
+                for (int q = 0; q + 1 < kP; ++q) {
+                    OX[a][q] = OX[a][q + 1];
+                    OU[a][q] = OU[a][q + 1];
+                }
+                OX[a][kP - 1] = ox[a];
+                OU[a][kP - 1] = ou[a];
+            }
+            oact = (oact >> 8) | (oa << (8 * (kP - 1)));
+        }
+        if (have) flush(h0, h1, h2, acc);
+        const bool full = vec_ok && s0 >= begin && s0 + kP <= end;
+        if (full) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                store_vec<kP>(out.x[a] + s0, OX[a]);
+                store_vec<kP>(out.u[a] + s0, OU[a]);
+            }
+            *reinterpret_cast<uint32_t*>(out.active + s0) = oact;
+        } else {
+#pragma unroll
+            for (int j = 0; j < kP; ++j) {
+                const int64_t i = s0 + j;
+                if (i < begin || i >= end) continue;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    out.x[a][i] = OX[a][j];
+                    out.u[a][i] = OU[a][j];
+                }
+                out.active[i] = (uint8_t)((oact >> (8 * j)) & 0xffu);
+            }
+        }
+    }
+    __syncthreads();
+    float* const J[3] = {Jx, Jy, Jz};
+    for (int e = threadIdx.x; e < 3 * kB3; e += kThreads) {
+        const float v = box[e];
+        if (v == 0.0f) continue;
+        const int c = e / kB3, r = e % kB3;
+        deposit_at(J[c], d, o[0] + r / (kB * kB), o[1] + (r / kB) % kB, o[2] + r % kB, v);
+    }
+}
+
+// The rare list of fused_tile_direct_kernel: each listed particle runs the
+// direct branch through the general route (global gather, FP64 anchors, the
+// 3^3 deposit into J).
+__global__ __launch_bounds__(kThreads) void fused_direct_rare_kernel(PicDesc d, PicSpecies sp, PicParticles in,
+                                                                     FieldPtrs fp, PicParticles out, float* Jx,
+                                                                     float* Jy, float* Jz,
+                                                                     const int32_t* __restrict__ rare) {
+    const int n = rare[0];
+    const GlobalSink sink{{Jx, Jy, Jz}};
+    for (int q = blockIdx.x * kThreads + threadIdx.x; q < n; q += gridDim.x * kThreads) {
+        const int64_t i = rare[1 + q];
+        const SpeciesK k = species_constants(d, sp, species_of(i, sp.slots, sp.count));
+        double pos[3];
+        load_pos(in, i, pos);
+        float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
+        float E[3], B[3];
+        CicAxis gc[3], gv[3];
+        for (int a = 0; a < 3; ++a) {
+            int an;
+            float dl;
+            cic_axis((float)pos[a], (float)d.org_c[a], (float)(1.0 / d.sp_c[a]), d.org_c[a], d.sp_c[a], d.dims[a],
+                     gc[a], an, dl);
+            cic_axis((float)pos[a], (float)d.org_v[a], (float)(1.0 / d.sp_v[a]), d.org_v[a], d.sp_v[a], d.dims[a],
+                     gv[a], an, dl);
+        }
+        E[0] = gather_cic(fp.E[0], fp.Ee[0], d, gv[0], gc[1], gc[2]);
+        E[1] = gather_cic(fp.E[1], fp.Ee[1], d, gc[0], gv[1], gc[2]);
+        E[2] = gather_cic(fp.E[2], fp.Ee[2], d, gc[0], gc[1], gv[2]);
+        B[0] = gather_cic(fp.B[0], fp.Be[0], d, gc[0], gv[1], gv[2]);
+        B[1] = gather_cic(fp.B[1], fp.Be[1], d, gv[0], gc[1], gv[2]);
+        B[2] = gather_cic(fp.B[2], fp.Be[2], d, gv[0], gv[1], gc[2]);
+        float nv[3] = {v[0], v[1], v[2]};
+        push_velocity(nv, E, B, k.h, d);
+        for (int a = 0; a < 3; ++a)
+            if ((k.uu >> a) & 1u) v[a] = nv[a];
+        bool act = true;
+        double xh[3];
+        for (int a = 0; a < 3; ++a) {
+            xh[a] = ((k.ux >> a) & 1u) ? (double)(float)step_pos(pos[a], (double)v[a], 0.5 * d.dt) : pos[a];
+            apply_axis_boundary(d.pbc[a], d.wind[a], xh[a], v[a], act);
+            xh[a] = (double)(float)xh[a];
+        }
+        if (act) {
+            direct_deposit(d, k.q_dv, xh, v, sink);
+            for (int a = 0; a < 3; ++a) {
+                double t2 = ((k.ux >> a) & 1u) ? (double)(float)step_pos(xh[a], (double)v[a], 0.5 * d.dt) : xh[a];
+                apply_axis_boundary(d.pbc[a], d.wind[a], t2, v[a], act);
+                xh[a] = t2;
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            out.x[a][i] = act ? (float)xh[a] : 0.0f;
+            out.u[a][i] = act ? v[a] : 0.0f;
+        }
+        out.active[i] = act ? 1 : 0;
+    }
+}
+
 // Slot-ordered launch for 1-D and 2-D first-order runs (and the general
 // path): each block accumulates J in an LDS box sized to its particles.
 template <int kRounds>
@@ -655,6 +946,52 @@ int pic_fused_push_esirkepov(const PicDesc* desc, const PicSpecies* sp, const Pi
                            *in, fp, *out, J[0], J[1], J[2], n);
     }
 #undef PIC_FUSED_LAUNCH
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                                const float* const E[3], const float* const B[3], const float* const E_ext[3],
+                                const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,
+                                const int32_t* starts, int32_t min_blocks, int32_t* workspace, void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && particles_ok(sp, in) && particles_ok(sp, out) && E && B && J && starts && workspace);
+    PIC_REQUIRE(fused_kind(*desc) == 3 && (tile == 4 || tile == 8) && (min_blocks == 1 || min_blocks == 4));
+    ExactGrid eg;
+    PIC_REQUIRE(make_exact_grid(*desc, eg));
+    for (int a = 0; a < 3; ++a) PIC_REQUIRE(desc->d[a] == desc->sp_c[a]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) PIC_REQUIRE(E[a] && B[a] && J[a] && in->x[a] != out->x[a] && in->u[a] != out->u[a]);
+    PIC_REQUIRE(in->active != out->active);
+    PIC_REQUIRE(field_size(*desc) < ((int64_t)1 << 31));
+    int64_t n = (int64_t)sp->count * sp->slots;
+    PIC_REQUIRE(n < ((int64_t)1 << 31) - 8 * kThreads);
+    if (n == 0) return 0;
+    TileGeom tg;
+    tg.T = tile;
+    for (int a = 0; a < 3; ++a) tg.nt[a] = (desc->n[a] + tile - 1) / tile;
+    tg.tiles = tg.nt[0] * tg.nt[1] * tg.nt[2];
+    const int64_t blocks = (int64_t)sp->count * (tg.tiles + 1);
+    PIC_REQUIRE(blocks < ((int64_t)1 << 31));
+    hipStream_t st = as_stream(stream);
+    FieldPtrs fp = make_fields(E, B, E_ext, B_ext);
+    int vec_ok = 1;
+    for (int a = 0; a < 3; ++a) vec_ok &= ((((uintptr_t)out->x[a]) | ((uintptr_t)out->u[a])) & 15u) == 0;
+    vec_ok &= (((uintptr_t)out->active) & 3u) == 0;
+    hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, workspace);
+#define PIC_DIRECT_LAUNCH(T, MB)                                                                                  \
+    hipLaunchKernelGGL((fused_tile_direct_kernel<T, MB>), dim3((unsigned)blocks), dim3(kThreads), 0, st, *desc, *sp, \
+                       *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace, vec_ok)
+    if (tile == 8) {
+        if (min_blocks == 4) PIC_DIRECT_LAUNCH(8, 4);
+        else PIC_DIRECT_LAUNCH(8, 1);
+    } else {
+        if (min_blocks == 4) PIC_DIRECT_LAUNCH(4, 4);
+        else PIC_DIRECT_LAUNCH(4, 1);
+    }
+#undef PIC_DIRECT_LAUNCH
+    const int64_t rare_grid = std::min<int64_t>(std::max<int64_t>(grid_1d(n, kThreads), 1), 1024);
+    hipLaunchKernelGGL(fused_direct_rare_kernel, dim3((unsigned)rare_grid), dim3(kThreads), 0, st, *desc, *sp, *in,
+                       fp, *out, J[0], J[1], J[2], workspace);
     PIC_CHECK_LAUNCH();
     return 0;
 }

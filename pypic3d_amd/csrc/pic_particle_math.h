@@ -363,6 +363,31 @@ __device__ inline void node_axis(double pos, double org, double sp, double dd, i
     s.k1 = 3;
 }
 
+// J_from_rhov deposit (J_from_rhov.py:88-200) of a particle at pos with
+// velocity u into a sink (add(d, c, i, j, k, v)): node weights along the other
+// axes, face weights along the current's axis, dqv = q w / (dx dy dz).
+template <class Sink>
+__device__ inline void direct_deposit(const PicDesc& d, float dqv, const double pos[3], const float u[3],
+                                      const Sink& sink) {
+    NodeAxis ax[3];
+    for (int a = 0; a < 3; ++a)
+        node_axis(pos[a], d.org_c[a], d.sp_c[a], d.d[a], d.shape_factor, d.n[a] == 1, d.g, ax[a]);
+    const float q[3] = {dqv * u[0], dqv * u[1], dqv * u[2]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (a < ax[0].k0 || a >= ax[0].k1 || b < ax[1].k0 || b >= ax[1].k1 || c < ax[2].k0 || c >= ax[2].k1)
+                    continue;
+                const int ix = ax[0].p[a], iy = ax[1].p[b], iz = ax[2].p[c];
+                sink.add(d, 0, ix, iy, iz, q[0] * ax[0].wf[a] * ax[1].wn[b] * ax[2].wn[c]);
+                sink.add(d, 1, ix, iy, iz, q[1] * ax[0].wn[a] * ax[1].wf[b] * ax[2].wn[c]);
+                sink.add(d, 2, ix, iy, iz, q[2] * ax[0].wn[a] * ax[1].wn[b] * ax[2].wf[c]);
+            }
+}
+
 __device__ inline void load_pos(const PicParticles& P, int64_t i, double x[3]) {
     x[0] = (double)P.x[0][i];
     x[1] = (double)P.x[1][i];

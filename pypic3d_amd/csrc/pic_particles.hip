@@ -124,23 +124,8 @@ __device__ inline void direct_particle(const PicDesc& d, float dqv, const PicPar
                                        const Sink& sink) {
     double pos[3];
     load_pos(in, i, pos);
-    NodeAxis ax[3];
-    for (int a = 0; a < 3; ++a)
-        node_axis(pos[a], d.org_c[a], d.sp_c[a], d.d[a], d.shape_factor, d.n[a] == 1, d.g, ax[a]);
-    const float q[3] = {dqv * in.u[0][i], dqv * in.u[1][i], dqv * in.u[2][i]};
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                if (a < ax[0].k0 || a >= ax[0].k1 || b < ax[1].k0 || b >= ax[1].k1 || c < ax[2].k0 || c >= ax[2].k1)
-                    continue;
-                const int ix = ax[0].p[a], iy = ax[1].p[b], iz = ax[2].p[c];
-                sink.add(d, 0, ix, iy, iz, q[0] * ax[0].wf[a] * ax[1].wn[b] * ax[2].wn[c]);
-                sink.add(d, 1, ix, iy, iz, q[1] * ax[0].wn[a] * ax[1].wf[b] * ax[2].wn[c]);
-                sink.add(d, 2, ix, iy, iz, q[2] * ax[0].wn[a] * ax[1].wn[b] * ax[2].wf[c]);
-            }
+    const float u[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
+    direct_deposit(d, dqv, pos, u, sink);
 }
 
 // compute_rho deposit of particle i (rho.py:72-153) into component 0 of a sink.

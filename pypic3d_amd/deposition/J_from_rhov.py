@@ -20,6 +20,12 @@ def deposit_direct_into(desc, sp, pp, J3, current_filter, stream, scratch=None, 
         N.call("pic_deposit_direct_tiled", desc, sp, pp, N.p3(J3), int(tile), N.ptr(starts), stream)
     else:
         N.call("pic_deposit_direct", desc, sp, pp, N.p3(J3), stream)
+    finish_direct_current(desc, J3, current_filter, stream, scratch)
+
+
+def finish_direct_current(desc, J3, current_filter, stream, scratch=None):
+    """Fold + refresh of the raw direct deposit, then the optional filter
+    (J_from_rhov.py:224-258)."""
     for c in J3:
         N.call("pic_halo_fold", desc, N.ptr(c), 1, stream)
         N.call("pic_halo_refresh", desc, N.ptr(c), 1, stream)

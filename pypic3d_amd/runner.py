@@ -16,7 +16,7 @@ import torch
 from . import _native as N
 from ._tensors import as_field3, check_fields, make_desc, make_species, particle_planes, pic_particles, soa_view
 from .deposition.Esirkepov import finish_current
-from .deposition.J_from_rhov import deposit_direct_into
+from .deposition.J_from_rhov import deposit_direct_into, finish_direct_current
 from .particles.particle_class import TiledParticles
 from .solvers.first_order_yee import _check_pml, _pml_args, yee_b, yee_e
 from .sorting import key_bits
@@ -30,8 +30,9 @@ class ElectrodynamicRunner:
                  tile=None, rounds=4, min_blocks=1):
         """tile (4 or 8): keep particles in tile-major cell order (sorted once
         here, then every ``sort_every`` steps) and use the tile-ordered kernels:
-        the fused push + Esirkepov kernel (3-D first order) or the direct
-        deposit of the J_from_rhov branch."""
+        the fused push + Esirkepov kernel (3-D first order); in the
+        J_from_rhov branch the fused push + direct-deposit kernel on exact 3-D
+        first-order grids, else the tiled direct deposit."""
         E, B, J, rho, phi, external_fields, pml_state, overflow = fields
         _check_pml(pml_state)
         if static_parameters.particle_pusher not in ("boris", "higuera_cary"):
@@ -78,7 +79,7 @@ class ElectrodynamicRunner:
         self.keys = None
         self.sorts_done = 0
         self.tile, self.rounds, self.min_blocks, self.starts = None, int(rounds), int(min_blocks), None
-        self.tile_ws = None
+        self.tile_ws, self.direct_fused = None, False
         if tile is not None:
             self.set_tile(tile)
 
@@ -126,6 +127,16 @@ class ElectrodynamicRunner:
                 ev[1].record()
             ev = self._mark("current_fold")
             finish_current(d, self.J, s)
+        elif self.tile and self.direct_fused:
+            for c in self.J:
+                N.call("pic_fill_f32", N.ptr(c), c.numel(), 0.0, s)
+            ev = self._mark("particles")
+            N.call("pic_fused_push_direct_tiled", d, self.sp, self._pp(src), N.p3(E), N.p3(self.B), self.Ee, self.Be,
+                   self._pp(dst), N.p3(self.J), self.tile, N.ptr(self.starts), 1, N.ptr(self.tile_ws), s)
+            if ev:
+                ev[1].record()
+            ev = self._mark("current_fold")
+            finish_direct_current(d, self.J, self.filter, s, scratch=self.scratch)
         else:
             ev = self._mark("particles")
             uo = [self.vel[dst][0], self.vel[dst][1], self.vel[dst][2]]
@@ -190,7 +201,7 @@ class ElectrodynamicRunner:
         """Switch to the tile-ordered kernel with `tile`^3-cell tiles (None:
         the slot-ordered kernel) and sort the particles accordingly."""
         if tile is None:
-            self.tile, self.starts, self.tile_ws = None, None, None
+            self.tile, self.starts, self.tile_ws, self.direct_fused = None, None, None, False
             return
         d = self.desc
         if self.esirkepov and not (d.shape_factor == 1 and all(d.n[a] > 1 for a in range(3))):
@@ -198,6 +209,11 @@ class ElectrodynamicRunner:
         if tile not in (4, 8):
             raise ValueError("tile must be 4 or 8")
         self.tile = int(tile)
+        # direct branch: one fused tile kernel on exact 3-D first-order grids with cells equal to the node
+        # spacing, the separate push / advance / boundary / tiled-deposit kernels otherwise
+        self.direct_fused = (not self.esirkepov and d.shape_factor == 1 and all(d.n[a] > 1 for a in range(3))
+                             and bool(N.lib().pic_tile_exact_grid(d))
+                             and all(d.d[a] == d.sp_c[a] for a in range(3)))
         tiles = int(N.lib().pic_tile_count(d, self.tile))
         self.starts = torch.empty(self.S * (tiles + 1) + 1, dtype=torch.int32, device=self.device)
         # the tiled kernel's list of particles for its general route (1 + slots int32)
