@@ -101,7 +101,18 @@ def extras(n, ppc, steps, seed):
         del parts
         ms = _timed(r.step, steps)
         npart = r.S * r.P
-        out[f"j_from_rhov_{filt}"] = {"ms_per_step": ms, "particle_steps_per_s": npart / (ms * 1e-3)}
+        r.enable_timing(True)
+        for _ in range(2):
+            r.step()
+        torch.cuda.synchronize()
+        ph = r.phase_ms()
+        r.enable_timing(False)
+        # particle phase (the fused direct tile kernel on this exact grid): particle planes read and
+        # written (50 B/particle), E and B gathered (24 B/cell), J written (12 B/cell)
+        out[f"j_from_rhov_{filt}"] = {"ms_per_step": ms, "particle_steps_per_s": npart / (ms * 1e-3),
+                                      "fused_kernel": bool(r.direct_fused), "phases_ms": ph,
+                                      "particle_phase_algorithmic_gbs":
+                                          (50 * npart + 36 * n_cells) / (ph["particles"] * 1e-3) / 1e9}
         if filt == "none":
             rho = torch.empty_like(r.J[0])
             ms_rho = _timed(lambda: deposit_rho_into(r.desc, r.sp, r._pp(r.cur), rho, "none", N.stream_handle(),
