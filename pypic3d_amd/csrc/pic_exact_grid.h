@@ -42,6 +42,14 @@ __device__ inline void exact_anchors(float x, float org, float sp, float isp, fl
     dv = upper ? x - mid : x - (mid - sp);
 }
 
+// An empty asm that reads a slot's position and velocity: placed before the
+// slot's activity test it keeps the compiler from moving these six loads below
+// the test, so they are issued with the activity load and share its memory
+// round trip instead of waiting for it.
+__device__ inline void loads_together(const float (&x)[3], const float (&u)[3]) {
+    asm volatile("" ::"v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(u[0]), "v"(u[1]), "v"(u[2]));
+}
+
 // Host check for ExactGrid (see its comment); false when any axis fails.
 bool make_exact_grid(const PicDesc& d, ExactGrid& eg) {
     for (int a = 0; a < 3; ++a) {

@@ -391,6 +391,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
             const bool act0 = in.active[i] != 0;
             const float xf[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
             float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
+            loads_together(xf, v);
             if (!act0) break;
             // ---- anchors on both node lines (grid_and_stencil.py:92-166), exact in FP32
             int anc[3], anv[3];
@@ -679,9 +680,11 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
             uint32_t oa = 0u;
             do {
             if (i < begin || i >= end) break;
-            if (!in.active[i]) break;
+            const bool act0 = in.active[i] != 0;
             const float xf[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
             float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
+            loads_together(xf, v);
+            if (!act0) break;
             int anc[3], anv[3];
             float dc[3], dv[3], nodec[3];
             bool fast = true;
