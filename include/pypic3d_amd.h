@@ -211,8 +211,11 @@ int pic_cell_keys(const PicDesc* desc, const PicSpecies* sp, const PicParticles*
                   int32_t* keys, void* stream);
 /* Tile-major key for the tile-ordered fused kernel (a reordering for
  * locality only; results do not depend on particle order): the cell
- * (ix, iy, iz) of pic_cell_keys, tiles of tile^3 cells in x-major order,
- * cells x-major inside a tile, inactive slots last in their species. */
+ * (ix, iy, iz) = clip(floor((x + W/2) * (1/d)), 0, N-1) in FP64 (the product
+ * with the reciprocal can differ from pic_cell_keys' quotient for a position
+ * within an ulp of a cell face), tiles of tile^3 cells in x-major order,
+ * cells x-major inside a tile, inactive slots last in their species.
+ * count * slots < 2^31. */
 int64_t pic_tile_count(const PicDesc* desc, int32_t tile);
 int pic_tile_keys(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, int32_t tile, int32_t* keys,
                   void* stream);
@@ -225,7 +228,7 @@ int64_t pic_sort_workspace_bytes(int64_t n);
  * keys_sorted receives the sorted keys. */
 int pic_sort_permutation(const int32_t* keys, int32_t* keys_sorted, int32_t* perm, int64_t n,
                          int32_t key_bits, void* workspace, int64_t workspace_bytes, void* stream);
-/* out planes[i] = in planes[perm[i]] (7 planes: x,y,z,ux,uy,uz,active). */
+/* out planes[i] = in planes[perm[i]] (7 planes: x,y,z,ux,uy,uz,active); n < 2^31. */
 int pic_permute_particles(const PicParticles* in, PicParticles* out, const int32_t* perm,
                           int64_t n, void* stream);
 

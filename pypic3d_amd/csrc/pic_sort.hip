@@ -42,27 +42,39 @@ __host__ __device__ inline void tile_counts(const PicDesc& d, int T, int nt[3]) 
     for (int a = 0; a < 3; ++a) nt[a] = (d.n[a] + T - 1) / T;
 }
 
-__global__ void tile_keys_kernel(PicDesc d, PicSpecies sp, PicParticles in, int T, uint32_t* keys, int64_t n) {
+// Per-axis constants of the tile key: W/2 and 1/d in FP64, the tile edge as a
+// shift when it is a power of two (else -1).
+struct TileKeyConst {
+    double half_w[3], inv_d[3];
+    int nt[3], shift;
+};
+
+__global__ void tile_keys_kernel(PicDesc d, PicSpecies sp, PicParticles in, int T, TileKeyConst kc, uint32_t* keys,
+                                 int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const uint32_t u = (uint32_t)i;     // n < 2^31 is checked at launch
     int s = species_of(i, sp.slots, sp.count);
-    int nt[3];
-    tile_counts(d, T, nt);
     const int64_t vol = (int64_t)T * T * T;
-    const int64_t span = (int64_t)nt[0] * nt[1] * nt[2] * vol;
+    const int64_t span = (int64_t)kc.nt[0] * kc.nt[1] * kc.nt[2] * vol;
     int64_t key = span;
-    if (in.active[i]) {
-        int c[3];
+    const bool act = in.active[u] != 0;
+    const float x[3] = {in.x[0][u], in.x[1][u], in.x[2][u]};
+    if (act) {
+        int c[3], tq[3], tr[3];
+#pragma unroll
         for (int a = 0; a < 3; ++a) {
-            double t = floor(((double)in.x[a][i] + 0.5 * d.wind[a]) / d.d[a]);
+            double t = floor(((double)x[a] + kc.half_w[a]) * kc.inv_d[a]);
             t = fmin(fmax(t, 0.0), (double)(d.n[a] - 1));
             c[a] = (int)t;
+            tq[a] = kc.shift >= 0 ? c[a] >> kc.shift : c[a] / T;
+            tr[a] = kc.shift >= 0 ? c[a] & (T - 1) : c[a] % T;
         }
-        int64_t tile = ((int64_t)(c[0] / T) * nt[1] + c[1] / T) * nt[2] + c[2] / T;
-        int64_t local = ((int64_t)(c[0] % T) * T + c[1] % T) * T + c[2] % T;
+        const int64_t tile = ((int64_t)tq[0] * kc.nt[1] + tq[1]) * kc.nt[2] + tq[2];
+        const int64_t local = ((int64_t)tr[0] * T + tr[1]) * T + tr[2];
         key = tile * vol + local;
     }
-    keys[i] = (uint32_t)(s * (span + 1) + key);
+    keys[u] = (uint32_t)(s * (span + 1) + key);
 }
 
 // starts[b] = first sorted position whose key >= the first key of tile b,
@@ -125,9 +137,19 @@ int pic_tile_keys(const PicDesc* desc, const PicSpecies* sp, const PicParticles*
     const int64_t span = pic_tile_count(desc, tile) * tile * tile * tile;
     PIC_REQUIRE((int64_t)sp->count * (span + 1) < ((int64_t)1 << 31));
     int64_t n = (int64_t)sp->count * sp->slots;
+    PIC_REQUIRE(n < ((int64_t)1 << 31));
     if (n == 0) return 0;
+    TileKeyConst kc;
+    tile_counts(*desc, tile, kc.nt);
+    kc.shift = -1;
+    for (int b = 0; b < 31; ++b)
+        if (tile == (1 << b)) kc.shift = b;
+    for (int a = 0; a < 3; ++a) {
+        kc.half_w[a] = 0.5 * desc->wind[a];
+        kc.inv_d[a] = 1.0 / desc->d[a];
+    }
     hipLaunchKernelGGL(tile_keys_kernel, dim3(grid_1d(n, kBlock)), dim3(kBlock), 0, as_stream(stream), *desc, *sp, *in,
-                       (int)tile, reinterpret_cast<uint32_t*>(keys), n);
+                       (int)tile, kc, reinterpret_cast<uint32_t*>(keys), n);
     PIC_CHECK_LAUNCH();
     return 0;
 }
@@ -186,6 +208,7 @@ int pic_permute_particles(const PicParticles* in, PicParticles* out, const int32
     PIC_REQUIRE(in && out && perm && n >= 0);
     for (int a = 0; a < 3; ++a) PIC_REQUIRE(in->x[a] && in->u[a] && out->x[a] && out->u[a] && in->x[a] != out->x[a]);
     PIC_REQUIRE(in->active && out->active && in->active != out->active);
+    PIC_REQUIRE(n < ((int64_t)1 << 31));
     if (n == 0) return 0;
     hipLaunchKernelGGL(permute_kernel, dim3(grid_1d(n, kBlock)), dim3(kBlock), 0, as_stream(stream), *in, *out, perm,
                        n);
