@@ -217,6 +217,49 @@ def test_cell_keys_and_stable_sort_are_bit_exact():
     np.testing.assert_array_equal(flat(xs), flat(x)[O.sort_permutation(want.reshape(-1))])
 
 
+@pytest.mark.parametrize("tile", [4, 8, 3])
+def test_tile_keys_and_ranges_are_exact(tile):
+    """pic_tile_keys: key = s*(span+1) + tile*T^3 + cell-in-tile with the
+    cell clip(floor((x + W/2) * (1/d)), 0, N-1) in FP64 (the same operations
+    in NumPy give the same bits), inactive slots at s*(span+1)+span; the tile
+    ranges are the left insertion points of each (species, tile) key."""
+    from pypic3d_amd import _native as N
+    from pypic3d_amd._tensors import make_desc, make_species, particle_planes, pic_particles
+    static, dyn, p = _setup((18, 12, 9), wind=(9.0, 6.0, 4.5))
+    S, P = 2, 6000
+    x, u, act = random_particles(p, S=S, P=P, seed=11, inactive_frac=0.05)
+    x[0, :50, 0] = np.round(x[0, :50, 0] * 2.0) / 2.0            # on cell faces
+    x[1, :20, 1] = p.wind[1] / 2                                  # on the +W/2 edge (clipped)
+    x[1, 20:40, 2] = -p.wind[2] / 2
+    parts = tiled_particles_from_arrays(x, u, act)
+    xp, up, ap = particle_planes(parts)
+    desc, sp = make_desc(static, dyn), make_species(_cfg(species(S)), P)
+    keys = torch.empty(S * P, dtype=torch.int32, device="cuda")
+    N.call("pic_tile_keys", desc, sp, pic_particles(xp, up, ap), tile, N.ptr(keys), N.stream_handle())
+    n = np.array(p.N)
+    nt = (n + tile - 1) // tile
+    vol = tile ** 3
+    span = int(nt.prod()) * vol
+    c = np.empty((S, P, 3), dtype=np.int64)
+    for a in range(3):
+        t = np.floor((x[..., a].astype(np.float32).astype(np.float64) + 0.5 * p.wind[a]) * (1.0 / p.d[a]))
+        c[..., a] = np.clip(t, 0, n[a] - 1)
+    tl = ((c[..., 0] // tile) * nt[1] + c[..., 1] // tile) * nt[2] + c[..., 2] // tile
+    loc = ((c[..., 0] % tile) * tile + c[..., 1] % tile) * tile + c[..., 2] % tile
+    want = np.where(act, tl * vol + loc, span) + np.arange(S)[:, None] * (span + 1)
+    np.testing.assert_array_equal(keys.cpu().numpy().reshape(S, P), want)
+    # ranges on the sorted keys
+    tiles = int(N.lib().pic_tile_count(desc, tile))
+    assert tiles == int(nt.prod())
+    ks = torch.as_tensor(np.sort(want.reshape(-1)).astype(np.int32)).cuda()
+    starts = torch.empty(S * (tiles + 1) + 1, dtype=torch.int32, device="cuda")
+    N.call("pic_tile_ranges", desc, sp, tile, N.ptr(ks), N.ptr(starts), N.stream_handle())
+    b = np.arange(S * (tiles + 1))
+    targets = (b // (tiles + 1)) * (span + 1) + (b % (tiles + 1)) * vol
+    want_starts = np.append(np.searchsorted(np.sort(want.reshape(-1)), targets, side="left"), S * P)
+    np.testing.assert_array_equal(starts.cpu().numpy(), want_starts)
+
+
 def test_poisson_cg_single_mode():
     static, dyn, p = _setup((24, 20, 16), wind=(2 * np.pi,) * 3, electrostatic=True)
     xs = [O.tiled_axis(p, a, "center")[2:-2] for a in range(3)]

@@ -22,11 +22,11 @@ from tests.helpers import random_field
 pytestmark = pytest.mark.gpu
 
 
-def _runner(N_, wind, pbc, tile, dt=None, seed=7, d=None):
+def _runner(N_, wind, pbc, tile, dt=None, seed=7, d=None, specs=None):
     static, dyn = workloads.make_parameters(N_, wind=wind, d=d, cfl=0.5, dt=dt, pbc=pbc,
                                             current_deposition="direct")
-    specs = [workloads.SpeciesSpec(20, weight=0.01, vth=(0.12,) * 3),
-             workloads.SpeciesSpec(6, charge=1.0, mass=20.0, weight=0.03, vth=(0.05,) * 3)]
+    specs = specs or [workloads.SpeciesSpec(20, weight=0.01, vth=(0.12,) * 3),
+                      workloads.SpeciesSpec(6, charge=1.0, mass=20.0, weight=0.03, vth=(0.05,) * 3)]
     parts, cfg = workloads.plasma(static, dyn, specs, seed=seed)
     p = oracle_params(static, dyn)
     fields = workloads.empty_fields(static, dyn)
@@ -97,6 +97,26 @@ def test_fused_direct_launch_matches_separate_kernels(tile, pbc, dt, steps):
     assert float((u0 - u1).abs().max()) <= 2e-6 * float(u0.abs().max())
     if pbc[1] == 2:
         assert int(a0.sum()) < n     # the absorbing wall removed particles
+    assert float((J0 - J1).norm() / J0.norm()) <= 1e-6
+
+
+def test_fused_direct_with_frozen_ions_and_drifting_beam():
+    """Three species: a drifting beam, a thermal species and frozen ions
+    (update_x = update_u = False, no push and no motion, but a deposit)."""
+    specs = [workloads.SpeciesSpec(8, weight=0.01, vth=(0.02,) * 3, drift=(0.3, -0.1, 0.05)),
+             workloads.SpeciesSpec(8, weight=0.01, vth=(0.1,) * 3),
+             workloads.SpeciesSpec(4, charge=1.0, mass=1836.0, weight=0.04, vth=(0.01,) * 3,
+                                   update_x=False, update_u=False)]
+    r = _runner((16, 12, 8), (16.0, 12.0, 8.0), (0, 0, 0), 4, specs=specs)
+    assert r.direct_fused
+    for _ in range(3):
+        r.step()
+    J1, x1, u1, a1, nr = _fused(r)
+    J0, x0, u0, a0 = _separate(r)
+    assert torch.equal(a0, a1)
+    assert float((x0 - x1).abs().max()) <= 1e-6 * 16.0
+    assert float((u0 - u1).abs().max()) <= 2e-6 * float(u0.abs().max())
+    assert torch.equal(x1[:, 2], r.pos[r.cur][:, 2]) and torch.equal(u1[:, 2], r.vel[r.cur][:, 2])
     assert float((J0 - J1).norm() / J0.norm()) <= 1e-6
 
 
