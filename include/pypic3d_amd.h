@@ -53,7 +53,9 @@ typedef struct PicDesc {
     int32_t pad1;          /* 1 Higuera-Cary (higuera_cary.py:57-113)         */
 } PicDesc;
 
-/* SpeciesConfig (particles/particle_class.py:5-14). */
+/* SpeciesConfig (particles/particle_class.py:5-14).  count may be 0 and
+ * slots may be 0 (an empty set, as in the reference's particle-free runs):
+ * its planes may then be null and the particle entry points do nothing. */
 typedef struct PicSpecies {
     int32_t count;
     int32_t slots;                    /* P: slots per species               */

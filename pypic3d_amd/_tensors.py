@@ -114,8 +114,8 @@ def make_species(species_config, slots: int) -> N.PicSpecies:
     sp = N.PicSpecies()
     charge = np.asarray(_np(species_config.charge), dtype=np.float64).reshape(-1)
     S = charge.shape[0]
-    if S < 1 or S > N.MAX_SPECIES:
-        raise ValueError(f"species count {S} outside [1, {N.MAX_SPECIES}]")
+    if S > N.MAX_SPECIES:
+        raise ValueError(f"species count {S} above {N.MAX_SPECIES}")
     mass = np.asarray(_np(species_config.mass), dtype=np.float64).reshape(-1)
     weight = np.asarray(_np(species_config.weight), dtype=np.float64).reshape(-1)
     ux = np.asarray(_np(species_config.update_x), dtype=bool).reshape(S, 3)

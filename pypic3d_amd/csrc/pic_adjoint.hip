@@ -1061,9 +1061,9 @@ int pic_adjoint_particles(const PicDesc* desc, const PicSpecies* sp, const PicPa
                           int32_t* workspace, void* stream) {
     PIC_REQUIRE(desc_ok(desc) && periodic_all(desc) && particles_ok(sp, state) && E && B && lamJ && lam_x &&
                 lam_v && lamE && lamB && workspace);
-    for (int a = 0; a < 3; ++a)
-        PIC_REQUIRE(E[a] && B[a] && lamJ[a] && lam_x[a] && lam_v[a] && lamE[a] && lamB[a]);
     int64_t n = (int64_t)sp->count * sp->slots;
+    for (int a = 0; a < 3; ++a)
+        PIC_REQUIRE(E[a] && B[a] && lamJ[a] && ((lam_x[a] && lam_v[a]) || n == 0) && lamE[a] && lamB[a]);
     PIC_REQUIRE(n < ((int64_t)1 << 31));
     if (n == 0) return 0;
     FieldPtrs fp = make_fields(E, B, E_ext, B_ext);

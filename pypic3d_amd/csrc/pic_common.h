@@ -107,6 +107,17 @@ __device__ inline int pmod(int a, int L) {
     return r < 0 ? r + L : r;
 }
 
+// An empty set (no species or no slots) may come with null planes: the
+// entry points return before touching them.
+inline bool particles_ok(const PicSpecies* sp, const PicParticles* p) {
+    if (!sp || !p) return false;
+    if (sp->count < 0 || sp->count > PIC_MAX_SPECIES || sp->slots < 0) return false;
+    if (sp->count == 0 || sp->slots == 0) return true;
+    for (int a = 0; a < 3; ++a)
+        if (!p->x[a] || !p->u[a]) return false;
+    return p->active != nullptr;
+}
+
 // species of flat slot i (slots are species-major, S <= PIC_MAX_SPECIES)
 __device__ inline int species_of(int64_t i, int slots, int count) {
     int s = 0;

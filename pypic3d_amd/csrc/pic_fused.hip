@@ -922,9 +922,10 @@ int pic_fused_push_esirkepov(const PicDesc* desc, const PicSpecies* sp, const Pi
                              const float* const B_ext[3], PicParticles* out, float* const J[3], void* stream) {
     PIC_REQUIRE(desc_ok(desc) && particles_ok(sp, in) && particles_ok(sp, out) && E && B && J);
     PIC_REQUIRE(desc->n[0] > 1 || desc->n[1] > 1 || desc->n[2] > 1);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) PIC_REQUIRE(E[a] && B[a] && J[a] && in->x[a] != out->x[a] && in->u[a] != out->u[a]);
-    PIC_REQUIRE(in->active != out->active);
+    const bool empty = (int64_t)sp->count * sp->slots == 0;
+    for (int a = 0; a < 3; ++a)
+        PIC_REQUIRE(E[a] && B[a] && J[a] && (empty || (in->x[a] != out->x[a] && in->u[a] != out->u[a])));
+    PIC_REQUIRE(empty || in->active != out->active);
     PIC_REQUIRE(field_size(*desc) < ((int64_t)1 << 31));
     int64_t n = (int64_t)sp->count * sp->slots;
     PIC_REQUIRE(n < ((int64_t)1 << 31) - 8 * kThreads);
@@ -962,9 +963,10 @@ int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const
     ExactGrid eg;
     PIC_REQUIRE(make_exact_grid(*desc, eg));
     for (int a = 0; a < 3; ++a) PIC_REQUIRE(desc->d[a] == desc->sp_c[a]);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) PIC_REQUIRE(E[a] && B[a] && J[a] && in->x[a] != out->x[a] && in->u[a] != out->u[a]);
-    PIC_REQUIRE(in->active != out->active);
+    const bool empty = (int64_t)sp->count * sp->slots == 0;
+    for (int a = 0; a < 3; ++a)
+        PIC_REQUIRE(E[a] && B[a] && J[a] && (empty || (in->x[a] != out->x[a] && in->u[a] != out->u[a])));
+    PIC_REQUIRE(empty || in->active != out->active);
     PIC_REQUIRE(field_size(*desc) < ((int64_t)1 << 31));
     int64_t n = (int64_t)sp->count * sp->slots;
     PIC_REQUIRE(n < ((int64_t)1 << 31) - 8 * kThreads);
@@ -1012,9 +1014,10 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
     PIC_REQUIRE(desc_ok(desc) && particles_ok(sp, in) && particles_ok(sp, out) && E && B && J && starts);
     PIC_REQUIRE(fused_kind(*desc) == 3 && (tile == 4 || tile == 8) && (rounds == 2 || rounds == 4));
     PIC_REQUIRE(min_blocks == 1 || min_blocks == 3 || min_blocks == 4);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) PIC_REQUIRE(E[a] && B[a] && J[a] && in->x[a] != out->x[a] && in->u[a] != out->u[a]);
-    PIC_REQUIRE(in->active != out->active);
+    const bool empty = (int64_t)sp->count * sp->slots == 0;
+    for (int a = 0; a < 3; ++a)
+        PIC_REQUIRE(E[a] && B[a] && J[a] && (empty || (in->x[a] != out->x[a] && in->u[a] != out->u[a])));
+    PIC_REQUIRE(empty || in->active != out->active);
     PIC_REQUIRE(field_size(*desc) < ((int64_t)1 << 31));
     int64_t n = (int64_t)sp->count * sp->slots;
     PIC_REQUIRE(n < ((int64_t)1 << 31) - 8 * kThreads);

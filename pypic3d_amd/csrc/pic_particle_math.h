@@ -417,14 +417,6 @@ __device__ inline void apply_axis_boundary(int bc, double wind, double& x, float
 }
 
 
-inline bool particles_ok(const PicSpecies* sp, const PicParticles* p) {
-    if (!sp || !p) return false;
-    if (sp->count < 1 || sp->count > PIC_MAX_SPECIES || sp->slots < 0) return false;
-    for (int a = 0; a < 3; ++a)
-        if (!p->x[a] || !p->u[a]) return false;
-    return p->active != nullptr;
-}
-
 inline bool desc_ok(const PicDesc* d) {
     if (!d) return false;
     if (d->g < 1 || (d->shape_factor != 1 && d->shape_factor != 2)) return false;

@@ -359,8 +359,8 @@ int pic_push(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, 
              const float* const B[3], const float* const E_ext[3], const float* const B_ext[3], float* const u_out[3],
              void* stream) {
     PIC_REQUIRE(desc_ok(desc) && particles_ok(sp, in) && E && B && u_out);
-    for (int a = 0; a < 3; ++a) PIC_REQUIRE(E[a] && B[a] && u_out[a]);
     int64_t n = (int64_t)sp->count * sp->slots;
+    for (int a = 0; a < 3; ++a) PIC_REQUIRE(E[a] && B[a] && (u_out[a] || n == 0));
     if (n == 0) return 0;
     if (desc->shape_factor == 1)
         hipLaunchKernelGGL(push_kernel<true>, dim3(grid_1d(n, kBlock)), dim3(kBlock), 0, as_stream(stream), *desc, *sp,
@@ -374,9 +374,10 @@ int pic_push(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, 
 
 int pic_advance_positions(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, double dt_scale,
                           float* const x_out[3], void* stream) {
-    PIC_REQUIRE(desc_ok(desc) && particles_ok(sp, in) && x_out && x_out[0] && x_out[1] && x_out[2]);
+    PIC_REQUIRE(desc_ok(desc) && particles_ok(sp, in) && x_out);
     int64_t n = (int64_t)sp->count * sp->slots;
     if (n == 0) return 0;
+    PIC_REQUIRE(x_out[0] && x_out[1] && x_out[2]);
     hipLaunchKernelGGL(advance_kernel, dim3(grid_1d(n, kBlock)), dim3(kBlock), 0, as_stream(stream), *desc, *sp, *in,
                        desc->dt * dt_scale, x_out[0], x_out[1], x_out[2], n);
     PIC_CHECK_LAUNCH();

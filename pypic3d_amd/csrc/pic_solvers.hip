@@ -353,8 +353,8 @@ int64_t pic_energy_workspace_bytes(const PicDesc* desc, const PicSpecies* sp) {
 int pic_energy(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, const float* const E[3],
                const float* const B[3], double* out, void* workspace, int64_t workspace_bytes, void* stream) {
     PIC_REQUIRE(desc_ok(desc) && sp && in && E && B && out && workspace);
-    for (int a = 0; a < 3; ++a) PIC_REQUIRE(E[a] && B[a] && in->u[a]);
-    PIC_REQUIRE(in->active && sp->count >= 1 && sp->count <= PIC_MAX_SPECIES && sp->slots >= 0);
+    for (int a = 0; a < 3; ++a) PIC_REQUIRE(E[a] && B[a]);
+    PIC_REQUIRE(particles_ok(sp, in));
     PIC_REQUIRE(workspace_bytes >= pic_energy_workspace_bytes(desc, sp));
     hipStream_t s = as_stream(stream);
     double* fpart = static_cast<double*>(workspace);

@@ -132,8 +132,7 @@ int64_t pic_tile_count(const PicDesc* desc, int32_t tile) {
 
 int pic_tile_keys(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, int32_t tile, int32_t* keys,
                   void* stream) {
-    PIC_REQUIRE(desc && sp && in && keys && in->active && in->x[0] && in->x[1] && in->x[2] && tile >= 1);
-    PIC_REQUIRE(sp->count >= 1 && sp->count <= PIC_MAX_SPECIES);
+    PIC_REQUIRE(desc && keys && tile >= 1 && particles_ok(sp, in));
     const int64_t span = pic_tile_count(desc, tile) * tile * tile * tile;
     PIC_REQUIRE((int64_t)sp->count * (span + 1) < ((int64_t)1 << 31));
     int64_t n = (int64_t)sp->count * sp->slots;
@@ -156,7 +155,8 @@ int pic_tile_keys(const PicDesc* desc, const PicSpecies* sp, const PicParticles*
 
 int pic_tile_ranges(const PicDesc* desc, const PicSpecies* sp, int32_t tile, const int32_t* keys_sorted,
                     int32_t* starts, void* stream) {
-    PIC_REQUIRE(desc && sp && keys_sorted && starts && tile >= 1);
+    PIC_REQUIRE(desc && sp && starts && tile >= 1 && sp->count >= 0 && sp->count <= PIC_MAX_SPECIES);
+    PIC_REQUIRE(sp->slots >= 0 && (keys_sorted || (int64_t)sp->count * sp->slots == 0));
     const int64_t tiles = pic_tile_count(desc, tile);
     const int64_t vol = (int64_t)tile * tile * tile;
     const int64_t n = (int64_t)sp->count * sp->slots;
@@ -170,8 +170,7 @@ int pic_tile_ranges(const PicDesc* desc, const PicSpecies* sp, int32_t tile, con
 }
 
 int pic_cell_keys(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, int32_t* keys, void* stream) {
-    PIC_REQUIRE(desc && sp && in && keys && in->active && in->x[0] && in->x[1] && in->x[2]);
-    PIC_REQUIRE(sp->count >= 1 && sp->count <= PIC_MAX_SPECIES);
+    PIC_REQUIRE(desc && keys && particles_ok(sp, in));
     int64_t ncell = (int64_t)desc->n[0] * desc->n[1] * desc->n[2];
     PIC_REQUIRE((int64_t)sp->count * (ncell + 1) < ((int64_t)1 << 31));
     int64_t n = (int64_t)sp->count * sp->slots;
@@ -189,9 +188,10 @@ int64_t pic_sort_workspace_bytes(int64_t n) {
 
 int pic_sort_permutation(const int32_t* keys, int32_t* keys_sorted, int32_t* perm, int64_t n, int32_t key_bits,
                          void* workspace, int64_t workspace_bytes, void* stream) {
-    PIC_REQUIRE(keys && keys_sorted && perm && n >= 0 && key_bits >= 1 && key_bits <= 32);
+    PIC_REQUIRE(n >= 0 && key_bits >= 1 && key_bits <= 32);
     PIC_REQUIRE(n < ((int64_t)1 << 31));
     if (n == 0) return 0;
+    PIC_REQUIRE(keys && keys_sorted && perm);
     size_t need = sort_storage(n, key_bits);
     PIC_REQUIRE(workspace != nullptr && (size_t)workspace_bytes >= need);
     size_t bytes = need;
@@ -205,11 +205,12 @@ int pic_sort_permutation(const int32_t* keys, int32_t* keys_sorted, int32_t* per
 }
 
 int pic_permute_particles(const PicParticles* in, PicParticles* out, const int32_t* perm, int64_t n, void* stream) {
-    PIC_REQUIRE(in && out && perm && n >= 0);
+    PIC_REQUIRE(in && out && n >= 0);
+    if (n == 0) return 0;
+    PIC_REQUIRE(perm);
     for (int a = 0; a < 3; ++a) PIC_REQUIRE(in->x[a] && in->u[a] && out->x[a] && out->u[a] && in->x[a] != out->x[a]);
     PIC_REQUIRE(in->active && out->active && in->active != out->active);
     PIC_REQUIRE(n < ((int64_t)1 << 31));
-    if (n == 0) return 0;
     hipLaunchKernelGGL(permute_kernel, dim3(grid_1d(n, kBlock)), dim3(kBlock), 0, as_stream(stream), *in, *out, perm,
                        n);
     PIC_CHECK_LAUNCH();

@@ -1,0 +1,93 @@
+"""Empty particle sets (no slots, or no species), as the reference's
+particle-free runs use them (utils_test.py:286-316, pml_test.py:527-617):
+every particle entry point accepts them and does nothing, the deposits give
+zero currents and charge, and a step evolves the fields exactly as a step
+with the same fields and only inactive particles does."""
+
+import numpy as np
+import pytest
+import torch
+
+from pypic3d_amd import workloads
+from pypic3d_amd._tensors import tiled_particles_from_arrays
+from pypic3d_amd.deposition.Esirkepov import Esirkepov_current
+from pypic3d_amd.deposition.J_from_rhov import J_from_rhov
+from pypic3d_amd.deposition.rho import compute_rho
+from pypic3d_amd.evolve import time_loop_electrodynamic
+from pypic3d_amd.particles.particle_class import SpeciesConfig, TiledParticles
+from pypic3d_amd.particles.particle_tile_communication import (refresh_tiled_particle_tiles,
+                                                               update_tiled_particle_positions)
+from pypic3d_amd.pusher.particle_push import particle_push
+from pypic3d_amd.utils import add_external_fields, compute_energy
+from tests.gpu_helpers import gpu_field, oracle_params
+from tests.helpers import random_field
+
+pytestmark = pytest.mark.gpu
+
+
+def _empty(S, P=0):
+    z = torch.zeros((1, 1, 1, S, P, 3), dtype=torch.float32, device="cuda")
+    parts = TiledParticles(x=z, u=z.clone(), active=torch.zeros((1, 1, 1, S, P), dtype=torch.bool, device="cuda"))
+    cfg = SpeciesConfig(charge=np.full(S, -1.0), mass=np.ones(S), weight=np.ones(S),
+                        update_x=np.ones((S, 3), bool), update_u=np.ones((S, 3), bool))
+    return parts, cfg
+
+
+def test_energy_of_external_fields_with_no_particles():
+    """utils_test.py:286-316: 1^3 grid, eps = 2, mu = 4, C = 10, external
+    Ex = 2 and By = 3 on the interior: U_E = 4, U_B = 1.125, kinetic 0."""
+    static, dyn = workloads.make_parameters((1, 1, 1), d=(1.0, 1.0, 1.0), dt=0.1, C=10.0, eps=2.0, mu=4.0)
+    E, B, *_ = workloads.empty_fields(static, dyn)
+    g = static.guard_cells
+    ext_E = [torch.zeros_like(c) for c in E]
+    ext_B = [torch.zeros_like(c) for c in B]
+    ext_E[0][0, 0, 0, g, g, g] = 2.0
+    ext_B[1][0, 0, 0, g, g, g] = 3.0
+    tE, tB = add_external_fields(E, B, (tuple(ext_E), tuple(ext_B)))
+    for S in (1, 0):
+        parts, cfg = _empty(S)
+        ue, ub, ke = compute_energy(parts, tE, tB, static, dyn, species_config=cfg)
+        assert float(ue) == pytest.approx(4.0, rel=1e-12)
+        assert float(ub) == pytest.approx(1.125, rel=1e-12)
+        assert float(ke) == 0.0
+
+
+@pytest.mark.parametrize("S", [1, 0])
+def test_particle_operators_on_empty_sets(S):
+    static, dyn = workloads.make_parameters((8, 6, 4), cfl=0.5)
+    p = oracle_params(static, dyn)
+    E = tuple(gpu_field(random_field(p, 1 + c, scale=0.1)) for c in range(3))
+    B = tuple(gpu_field(random_field(p, 4 + c, scale=0.1)) for c in range(3))
+    _, _, J, rho, *_ = workloads.empty_fields(static, dyn)
+    J = tuple(torch.full_like(c, 7.0) for c in J)           # templates: the result must not keep these values
+    parts, cfg = _empty(S)
+    pushed = particle_push(parts, cfg, E, B, static, dyn)
+    assert pushed.u.shape == parts.u.shape
+    moved = update_tiled_particle_positions(parts, cfg, float(dyn.dt))
+    assert moved.x.shape == parts.x.shape
+    refreshed, overflow = refresh_tiled_particle_tiles(parts, static, dyn)
+    assert refreshed.active.shape == parts.active.shape and not bool(overflow)
+    for Jn in (Esirkepov_current(parts, cfg, J, static, dyn), J_from_rhov(parts, cfg, J, static, dyn)):
+        for c in Jn:
+            assert float(c.abs().max()) == 0.0
+    r = compute_rho(parts, cfg, torch.full_like(rho, 3.0), static, dyn)
+    assert float(r.abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("dep", ["esirkepov", "direct"])
+def test_particle_free_step_equals_step_with_inactive_particles(dep):
+    static, dyn = workloads.make_parameters((10, 8, 6), cfl=0.5, current_deposition=dep)
+    p = oracle_params(static, dyn)
+    E = tuple(gpu_field(random_field(p, 11 + c, scale=0.1)) for c in range(3))
+    B = tuple(gpu_field(random_field(p, 14 + c, scale=0.1)) for c in range(3))
+    fields = (E, B) + workloads.empty_fields(static, dyn)[2:]
+    parts, cfg = _empty(1)
+    x = np.zeros((1, 5, 3), np.float32)
+    inactive = tiled_particles_from_arrays(x, x, np.zeros((1, 5), bool))
+    _, f_empty = time_loop_electrodynamic(parts, cfg, fields, static, dyn)
+    _, f_inact = time_loop_electrodynamic(inactive, cfg, fields, static, dyn)
+    for k in (0, 1, 2):
+        for a, b in zip(f_empty[k], f_inact[k]):
+            assert torch.equal(a, b), k
+    assert float(max(c.abs().max() for c in f_empty[2])) == 0.0
+    assert float(max(c.abs().max() for c in f_empty[0])) > 0.0
