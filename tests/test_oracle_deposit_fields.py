@@ -290,6 +290,12 @@ def test_particle_boundaries():
     assert np.all(xo[0, 0] == 0) and np.all(uo[0, 2] == 0)
 
 
+def test_wrap_periodic_known_answers():
+    # grid_and_stencil_test.py:27-33: W = 4
+    x = np.array([-2.5, -2.0, 1.75, 2.0, 2.25, 6.5])
+    np.testing.assert_array_equal(O.wrap_periodic(x, 4.0), [1.5, -2.0, 1.75, 2.0, -1.75, -1.5])
+
+
 def test_energy_closed_forms():
     p = O.Params(N=(4, 3, 2), wind=(2.0, 1.5, 1.0), g=2, eps=2.0, mu=0.5, C=1.0)
     E = [np.full(p.shape, 1.0), np.zeros(p.shape), np.zeros(p.shape)]
