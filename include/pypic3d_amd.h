@@ -199,8 +199,10 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
  * pic_push, pic_advance_positions, pic_particle_boundaries,
  * pic_deposit_direct, pic_advance_positions, pic_particle_boundaries up to
  * FP32 rounding (an ulp in a velocity where the compiler fuses the push's
- * multiply-adds differently) and summation order in J.  tile 4 or 8, min_blocks 1 or 4; workspace as
- * for pic_fused_push_esirkepov_tiled. */
+ * multiply-adds differently) and summation order in J.  tile 4 or 8;
+ * min_blocks 1 (each thread's four slots stored with 16-byte stores) or 4
+ * (slots stored one by one, 4 resident blocks per CU at tile 4: the faster
+ * shape); workspace as for pic_fused_push_esirkepov_tiled. */
 int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
                                 const float* const E[3], const float* const B[3], const float* const E_ext[3],
                                 const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,

@@ -614,8 +614,11 @@ __global__ void zero_count_kernel(int32_t* c) { *c = 0; }
 // weights 1 - df/d, df/d with df = dn - d/2 on the 2x2x2 nodes of the centre
 // anchor, summed in registers over a thread's equal-anchor particles and
 // added into an LDS J box), x += v dt/2, boundaries, store.  Particles the
-// fast loop does not cover are listed for fused_direct_rare_kernel.
-template <int kT, int kMinBlocks>
+// fast loop does not cover are listed for fused_direct_rare_kernel.  kVec:
+// a thread's four slots are stored with one 16-byte store per plane at the
+// end (24 more registers); otherwise each slot is stored as it finishes,
+// which keeps the kernel at 128 VGPRs for 4 waves per SIMD.
+template <int kT, int kMinBlocks, bool kVec>
 __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel(
     PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, PicParticles out, float* Jx, float* Jy, float* Jz,
     const int32_t* __restrict__ starts, TileGeom tg, ExactGrid eg, int32_t* __restrict__ rare, int vec_ok) {
@@ -796,19 +799,29 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
                 oa = act ? 1u : 0u;
             }
             } while (false);
+            if constexpr (kVec) {
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
+                for (int a = 0; a < 3; ++a) {
 #pragma unroll
-                for (int q = 0; q + 1 < kP; ++q) {
-                    OX[a][q] = OX[a][q + 1];
-                    OU[a][q] = OU[a][q + 1];
+                    for (int q = 0; q + 1 < kP; ++q) {
+                        OX[a][q] = OX[a][q + 1];
+                        OU[a][q] = OU[a][q + 1];
+                    }
+                    OX[a][kP - 1] = ox[a];
+                    OU[a][kP - 1] = ou[a];
                 }
-                OX[a][kP - 1] = ox[a];
-                OU[a][kP - 1] = ou[a];
+                oact = (oact >> 8) | (oa << (8 * (kP - 1)));
+            } else if (i >= begin && i < end) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    out.x[a][i] = ox[a];
+                    out.u[a][i] = ou[a];
+                }
+                out.active[i] = (uint8_t)oa;
             }
-            oact = (oact >> 8) | (oa << (8 * (kP - 1)));
         }
         if (have) flush(h0, h1, h2, acc);
+        if constexpr (!kVec) continue;
         const bool full = vec_ok && s0 >= begin && s0 + kP <= end;
         if (full) {
 #pragma unroll
@@ -983,15 +996,15 @@ int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const
     for (int a = 0; a < 3; ++a) vec_ok &= ((((uintptr_t)out->x[a]) | ((uintptr_t)out->u[a])) & 15u) == 0;
     vec_ok &= (((uintptr_t)out->active) & 3u) == 0;
     hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, workspace);
-#define PIC_DIRECT_LAUNCH(T, MB)                                                                                  \
-    hipLaunchKernelGGL((fused_tile_direct_kernel<T, MB>), dim3((unsigned)blocks), dim3(kThreads), 0, st, *desc, *sp, \
-                       *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace, vec_ok)
+#define PIC_DIRECT_LAUNCH(T, MB, V)                                                                               \
+    hipLaunchKernelGGL((fused_tile_direct_kernel<T, MB, V>), dim3((unsigned)blocks), dim3(kThreads), 0, st, *desc,   \
+                       *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace, vec_ok)
     if (tile == 8) {
-        if (min_blocks == 4) PIC_DIRECT_LAUNCH(8, 4);
-        else PIC_DIRECT_LAUNCH(8, 1);
+        if (min_blocks == 4) PIC_DIRECT_LAUNCH(8, 4, false);
+        else PIC_DIRECT_LAUNCH(8, 1, true);
     } else {
-        if (min_blocks == 4) PIC_DIRECT_LAUNCH(4, 4);
-        else PIC_DIRECT_LAUNCH(4, 1);
+        if (min_blocks == 4) PIC_DIRECT_LAUNCH(4, 4, false);
+        else PIC_DIRECT_LAUNCH(4, 1, true);
     }
 #undef PIC_DIRECT_LAUNCH
     const int64_t rare_grid = std::min<int64_t>(std::max<int64_t>(grid_1d(n, kThreads), 1), 1024);

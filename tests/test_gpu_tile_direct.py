@@ -59,7 +59,7 @@ def _separate(r):
             r.act[dst].clone())
 
 
-def _fused(r):
+def _fused(r, min_blocks=1):
     src, dst = r.cur, 1 - r.cur
     for c in r.J:
         c.zero_()
@@ -68,7 +68,7 @@ def _fused(r):
     r.act[dst].fill_(True)
     r.tile_ws[0] = -7
     N.call("pic_fused_push_direct_tiled", r.desc, r.sp, r._pp(src), N.p3(r.E[r.ecur]), N.p3(r.B), r.Ee, r.Be,
-           r._pp(dst), N.p3(r.J), r.tile, N.ptr(r.starts), 1, N.ptr(r.tile_ws), N.stream_handle())
+           r._pp(dst), N.p3(r.J), r.tile, N.ptr(r.starts), min_blocks, N.ptr(r.tile_ws), N.stream_handle())
     torch.cuda.synchronize()
     return (torch.stack(r.J).clone(), r.pos[dst].clone(), r.vel[dst].clone(),
             r.act[dst].clone(), int(r.tile_ws[0]))
@@ -86,6 +86,9 @@ def test_fused_direct_launch_matches_separate_kernels(tile, pbc, dt, steps):
     for _ in range(steps):      # drift away from the sorted order (no re-sort)
         r.step()
     J1, x1, u1, a1, nr = _fused(r)
+    J4, x4, u4, a4, nr4 = _fused(r, 4)          # slot-by-slot stores, 4 blocks per CU: same arithmetic
+    assert torch.equal(x1, x4) and torch.equal(u1, u4) and torch.equal(a1, a4) and nr4 == nr
+    assert float((J4 - J1).norm() / J1.norm()) <= 1e-6
     J0, x0, u0, a0 = _separate(r)
     n = a0.numel()
     assert 0 < nr < n
