@@ -132,8 +132,7 @@ class ElectrodynamicRunner:
                 N.call("pic_fill_f32", N.ptr(c), c.numel(), 0.0, s)
             ev = self._mark("particles")
             N.call("pic_fused_push_direct_tiled", d, self.sp, self._pp(src), N.p3(E), N.p3(self.B), self.Ee, self.Be,
-                   self._pp(dst), N.p3(self.J), self.tile, N.ptr(self.starts), 4 if self.tile == 4 else 1,
-                   N.ptr(self.tile_ws), s)
+                   self._pp(dst), N.p3(self.J), self.tile, N.ptr(self.starts), 1, N.ptr(self.tile_ws), s)
             if ev:
                 ev[1].record()
             ev = self._mark("current_fold")
