@@ -201,8 +201,8 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
  * FP32 rounding (an ulp in a velocity where the compiler fuses the push's
  * multiply-adds differently) and summation order in J.  tile 4 or 8;
  * min_blocks 1 (each thread's four slots stored with 16-byte stores) or 4
- * (slots stored one by one for 4 resident blocks per CU at tile 4: about 5 %
- * faster at 256^3 x 32 ppc but 2.4x the HBM traffic, since the 4-byte
+ * (slots stored one by one for 4 resident blocks per CU at tile 4: about 4 %
+ * faster at 256^3 x 32 ppc but 2.1x the HBM traffic, since the 4-byte
  * stores at a 16-byte lane stride reach HBM as partial lines); workspace as
  * for pic_fused_push_esirkepov_tiled. */
 int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
