@@ -304,7 +304,7 @@ def adjoint(n=64, ppc=32, steps=200, every=20, seed=20261202):
             "checkpoint_every": every, "forward_ms_per_step": 1e3 * fwd,
             "forward_with_checkpoints_ms_per_step": 1e3 * (t1 - t0) / steps,
             "adjoint_ms_per_step": 1e3 * (t2 - t1) / steps, "adjoint_over_forward": (t2 - t1) / steps / fwd,
-            "loss": float(loss), "dloss_ddrift": float(gd), "dloss_damplitude": float(ga),
+            "loss": float(loss.detach()), "dloss_ddrift": float(gd), "dloss_damplitude": float(ga),
             "max_abs_dloss_dv0": float(gv.abs().max())}
 
 
