@@ -320,12 +320,43 @@ __device__ inline void boris_vjp(const double v[3], const double E[3], const dou
 // ----------------------------------------------------------------- gather
 // One axis of interpolate_field_to_particles (boris.py:127-258): stencil
 // points, weights and their x-derivatives (weights divide by the grid spacing).
-struct AdjAxis {
-    int p[3];
+// kN = 3 holds the reference's 3-point stencil of either shape; kN = 2 holds
+// only the two points a, a+1 that carry the first-order weights (the
+// reference's third point has weight 0 for CIC), for first-order callers.
+template <int kN>
+struct AdjAxisN {
+    int p[kN];
     int r0;                 // unwrapped array index of point 0 (the scatter box coordinate)
-    double w[3], dw[3];
+    double w[kN], dw[kN];
     int n;
 };
+using AdjAxis = AdjAxisN<3>;
+
+// First-order two-point axis: the nonzero points of the 3-point stencil in
+// the same order, so every sum over points adds the same terms in the same
+// order (the zero-weight terms of the 3-point form add exact zeros).
+__device__ inline void adj_gather_axis(double x, double org, double sp, int sf, int L, bool active, int g,
+                                       AdjAxisN<2>& s) {
+    (void)sf;               // first order only
+    const int a = anchor_of(x, org, sp, 1);
+    const double r = (x - node_coord(a, sp, org)) / sp;
+    if (active) {
+        s.p[0] = pmod(a, L);
+        s.p[1] = pmod(a + 1, L);
+        s.w[0] = 1.0 - r;
+        s.w[1] = r;
+        s.dw[0] = -1.0 / sp;
+        s.dw[1] = 1.0 / sp;
+        s.r0 = a;
+        s.n = 2;
+    } else {
+        s.p[0] = g;
+        s.r0 = g;
+        s.w[0] = (1.0 - r) + r;
+        s.dw[0] = 0.0;
+        s.n = 1;
+    }
+}
 
 __device__ inline void adj_gather_axis(double x, double org, double sp, int sf, int L, bool active, int g,
                                        AdjAxis& s) {
@@ -360,24 +391,27 @@ __device__ inline void adj_gather_axis(double x, double org, double sp, int sf, 
     }
 }
 
-struct AdjGrid {
-    AdjAxis c[3], v[3];
+template <int kN>
+struct AdjGridN {
+    AdjAxisN<kN> c[3], v[3];
 };
+using AdjGrid = AdjGridN<3>;
 
 // value and d/dx of one staggered component; `stag` bit a set = vertex line on axis a
-__device__ inline void gather_grad(const PicDesc& d, const AdjGrid& gr, int stag, const float* F, const float* Fe,
-                                   double& val, double grad[3]) {
-    const AdjAxis& ax = (stag & 1) ? gr.v[0] : gr.c[0];
-    const AdjAxis& ay = (stag & 2) ? gr.v[1] : gr.c[1];
-    const AdjAxis& az = (stag & 4) ? gr.v[2] : gr.c[2];
+template <int kN>
+__device__ inline void gather_grad(const PicDesc& d, const AdjGridN<kN>& gr, int stag, const float* F,
+                                   const float* Fe, double& val, double grad[3]) {
+    const AdjAxisN<kN>& ax = (stag & 1) ? gr.v[0] : gr.c[0];
+    const AdjAxisN<kN>& ay = (stag & 2) ? gr.v[1] : gr.c[1];
+    const AdjAxisN<kN>& az = (stag & 4) ? gr.v[2] : gr.c[2];
     val = 0.0;
     grad[0] = grad[1] = grad[2] = 0.0;
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < kN; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
+        for (int j = 0; j < kN; ++j)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
+            for (int k = 0; k < kN; ++k) {
                 if (i >= ax.n || j >= ay.n || k >= az.n) continue;
                 int64_t id = fidx(d, ax.p[i], ay.p[j], az.p[k]);
                 double f = (double)F[id];
@@ -398,18 +432,19 @@ struct ScatterBox {
     bool on;
 };
 
-__device__ inline void scatter_adj(const PicDesc& d, const AdjGrid& gr, int stag, double lam, double* G,
+template <int kN>
+__device__ inline void scatter_adj(const PicDesc& d, const AdjGridN<kN>& gr, int stag, double lam, double* G,
                                    const ScatterBox& sb, int comp) {
     if (lam == 0.0) return;
-    const AdjAxis& ax = (stag & 1) ? gr.v[0] : gr.c[0];
-    const AdjAxis& ay = (stag & 2) ? gr.v[1] : gr.c[1];
-    const AdjAxis& az = (stag & 4) ? gr.v[2] : gr.c[2];
+    const AdjAxisN<kN>& ax = (stag & 1) ? gr.v[0] : gr.c[0];
+    const AdjAxisN<kN>& ay = (stag & 2) ? gr.v[1] : gr.c[1];
+    const AdjAxisN<kN>& az = (stag & 4) ? gr.v[2] : gr.c[2];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < kN; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
+        for (int j = 0; j < kN; ++j)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
+            for (int k = 0; k < kN; ++k) {
                 if (i >= ax.n || j >= ay.n || k >= az.n) continue;
                 double w = ax.w[i] * ay.w[j] * az.w[k];
                 if (w == 0.0) continue;
@@ -689,9 +724,10 @@ struct LamPtrs {
 constexpr int kAdjThreads = 128;
 
 // Gather values (no derivatives) of the six components at x.
+template <int kN>
 __device__ inline void gather_values(const PicDesc& d, const FieldPtrs& fp, const double x[3], double Ep[3],
                                      double Bp[3]) {
-    AdjGrid gr;
+    AdjGridN<kN> gr;
     for (int a = 0; a < 3; ++a) {
         bool act = d.n[a] > 1;
         adj_gather_axis(x[a], d.org_c[a], d.sp_c[a], d.shape_factor, d.dims[a], act, d.g, gr.c[a]);
@@ -752,7 +788,8 @@ __global__ __launch_bounds__(kAdjThreads, kMinBlocks) void adjoint_particles_ker
     }
     // ---- phase 1
     double Ep[3], Bp[3];
-    gather_values(d, fp, x, Ep, Bp);
+    constexpr int kPts = kAnalytic ? 2 : 3;   // the analytic path runs first-order stencils only
+    gather_values<kPts>(d, fp, x, Ep, Bp);
     double vp[3];
     {
         double vn[3];
@@ -823,7 +860,7 @@ __global__ __launch_bounds__(kAdjThreads, kMinBlocks) void adjoint_particles_ker
         lin[m] = acc;
     }
     // ---- phase 3: gather stencils, dE_p/dx, scatter box
-    AdjGrid gr;
+    AdjGridN<kPts> gr;
     for (int a = 0; a < 3; ++a) {
         bool act = d.n[a] > 1;
         adj_gather_axis(x[a], d.org_c[a], d.sp_c[a], d.shape_factor, d.dims[a], act, d.g, gr.c[a]);
