@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define PIC_ABI_VERSION 5
+#define PIC_ABI_VERSION 6
 #define PIC_MAX_SPECIES 16
 
 /* Scalar run description: StaticParameters + DynamicParameters fields the hot
@@ -314,15 +314,18 @@ int pic_adjoint_yee(const PicDesc* desc, double* const lamE[3], double* const la
  * lamJ: interior adjoint of J (read at each deposit node's periodic partner,
  * the transpose of fold + refresh); lamE, lamB: interior FP64 arrays to which
  * the adjoint of the gathered fields is added (atomics at the gather nodes'
- * periodic partners, the transpose of the ghost refresh).  min_blocks (1, 4 or
- * 8): occupancy hint for the register allocator.  workspace: 1 + count*slots
- * int32 (first-order particles that move two or more cells are finished by a
- * second, general pass). */
+ * periodic partners, the transpose of the ghost refresh).  min_blocks: kept
+ * for compatibility, not used (the first-order path's two kernels fit two
+ * waves per SIMD unforced).  workspace: 1 + count*slots int32 (first-order
+ * particles that move two or more cells are finished by a third, general
+ * pass).  scratch: 7*count*slots doubles between the first-order path's two
+ * kernels (push-Jacobian products and a done flag per slot); may be null for
+ * second-order shapes. */
 int pic_adjoint_particles(const PicDesc* desc, const PicSpecies* sp, const PicParticles* state,
                           const float* const E[3], const float* const B[3], const float* const E_ext[3],
                           const float* const B_ext[3], const double* const lamJ[3], double* const lam_x[3],
                           double* const lam_v[3], double* const lamE[3], double* const lamB[3], int32_t min_blocks,
-                          int32_t* workspace, void* stream);
+                          int32_t* workspace, double* scratch, void* stream);
 
 #ifdef __cplusplus
 }

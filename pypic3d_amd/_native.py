@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libpypic3d_hip.so")
 MAX_SPECIES = 16
 PUSHERS = {"boris": 0, "higuera_cary": 1}
 FUSED_VARIANTS = 6
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 c_int = ctypes.c_int
 c_i32 = ctypes.c_int32
@@ -108,7 +108,8 @@ SIGNATURES = {
     "pic_periodic_expand_f64": (c_int, [DP, c_vp, c_vp, c_vp]),
     "pic_periodic_fold_add_f64": (c_int, [DP, c_vp, c_vp, c_vp]),
     "pic_adjoint_yee": (c_int, [DP, P3, P3, P3, c_vp * 6, c_vp]),
-    "pic_adjoint_particles": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, P3, P3, P3, P3, P3, c_i32, c_vp, c_vp]),
+    "pic_adjoint_particles": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, P3, P3, P3, P3, P3, c_i32, c_vp, c_vp,
+                                      c_vp]),
 }
 
 _lib = None

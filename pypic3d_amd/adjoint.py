@@ -132,6 +132,7 @@ class AdjointRun:
         lx = torch.zeros((3, r.S, r.P), dtype=torch.float64, device=dev)
         lv = torch.zeros_like(lx)
         defer = torch.empty(1 + r.S * r.P, dtype=torch.int32, device=dev)
+        scratch = torch.empty(7 * r.S * r.P, dtype=torch.float64, device=dev)
         p3 = lambda t: N.P3(*[t[c].data_ptr() for c in range(3)])   # noqa: E731
         w6 = (N.c_vp * 6)(*[work[c].data_ptr() for c in range(6)])
         LE, LB, LJ, LX, LV = p3(lamE), p3(lamB), p3(lamJ), p3(lx), p3(lv)
@@ -151,7 +152,7 @@ class AdjointRun:
                                              [seg_v[m, 0], seg_v[m, 1], seg_v[m, 2]], seg_a)) for m in range(k)]
         seg_pE = [p3(seg_E[m]) for m in range(k)]
         seg_pB = [p3(seg_B[m]) for m in range(k)]
-        dref, spref, dfr = ctypes.byref(d), ctypes.byref(r.sp), N.ptr(defer)
+        dref, spref, dfr, scr = ctypes.byref(d), ctypes.byref(r.sp), N.ptr(defer), N.ptr(scratch)
         adj_yee, adj_part = lib.pic_adjoint_yee, lib.pic_adjoint_particles
         starts = sorted(self.ckpt)
         for si in reversed(range(len(starts))):
@@ -167,7 +168,7 @@ class AdjointRun:
             for m in reversed(range(n1 - n0)):
                 N.check(adj_yee(dref, LE, LB, LJ, w6, s), "pic_adjoint_yee")
                 N.check(adj_part(dref, spref, seg_pp[m], seg_pE[m], seg_pB[m], r.Ee, r.Be, LJ, LX, LV, LE, LB,
-                                 self.min_blocks, dfr, s), "pic_adjoint_particles")
+                                 self.min_blocks, dfr, scr, s), "pic_adjoint_particles")
         return lx, lv, lamE, lamB
 
 

@@ -752,10 +752,15 @@ __device__ inline void gather_values(const PicDesc& d, const FieldPtrs& fp, cons
 // nullptr) or over the `*count` slots listed in `only`.
 // kPush: 1 = Boris reverse sweep only, 2 = tangent passes only, 0 = chosen
 // per launch from the descriptor (both compiled in).
-template <bool kAnalytic, int kMinBlocks, int kPush = 0>
+// kPart: 0 = all three phases in one kernel; 1 = phases 1-2, writing lambda(x)
+// so far and lambda(v) into lp and lambda(E_p, B_p) with a done flag into
+// `scratch` (7 doubles per slot, plane-major); 2 = phase 3 for the flagged
+// slots.  The split keeps each kernel at two waves per SIMD (together they
+// need more than 256 registers).
+template <bool kAnalytic, int kMinBlocks, int kPush = 0, int kPart = 0>
 __global__ __launch_bounds__(kAdjThreads, kMinBlocks) void adjoint_particles_kernel(
     PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, LamPtrs lp, int64_t n, int32_t* defer,
-    int32_t* defer_count, const int32_t* only, const int32_t* only_count) {
+    int32_t* defer_count, const int32_t* only, const int32_t* only_count, double* __restrict__ scratch) {
     constexpr int kCap = 3072;   // doubles of LDS for the scatter box (6 components)
     __shared__ double sbox[kCap];
     __shared__ int bmin[3], bmax[3];
@@ -769,7 +774,8 @@ __global__ __launch_bounds__(kAdjThreads, kMinBlocks) void adjoint_particles_ker
     }
     __syncthreads();
     bool live = i < n && in.active[i];
-    if (i < n && !live) {
+    if constexpr (kPart == 2) live = i < n && scratch[6 * n + i] == 1.0;
+    if (kPart != 2 && i < n && !live) {
         for (int a = 0; a < 3; ++a) {
             lp.x[a][i] = 0.0;
             lp.v[a][i] = 0.0;
@@ -786,9 +792,11 @@ __global__ __launch_bounds__(kAdjThreads, kMinBlocks) void adjoint_particles_ker
         lx[a] = live ? lp.x[a][ii] : 0.0;   // adjoint of x' (the wrap is the identity map's derivative)
         lv[a] = live ? lp.v[a][ii] : 0.0;   // adjoint of v'
     }
+    constexpr int kPts = kAnalytic ? 2 : 3;   // the analytic path runs first-order stencils only
+    double lin[9];
+    if constexpr (kPart != 2) {
     // ---- phase 1
     double Ep[3], Bp[3];
-    constexpr int kPts = kAnalytic ? 2 : 3;   // the analytic path runs first-order stencils only
     gather_values<kPts>(d, fp, x, Ep, Bp);
     double vp[3];
     {
@@ -833,7 +841,6 @@ __global__ __launch_bounds__(kAdjThreads, kMinBlocks) void adjoint_particles_ker
         }
     }
     // ---- phase 2: lambda(v, E_p, B_p) = J_push^T lambda(v') on update_u axes
-    double lin[9];
     const bool vjp = kPush == 1 || (kPush == 0 && d.pusher == 0 && d.variant != 1);
     if (kPush != 2 && vjp) {
         double lo[3];
@@ -858,6 +865,23 @@ __global__ __launch_bounds__(kAdjThreads, kMinBlocks) void adjoint_particles_ker
         for (int a = 0; a < 3; ++a) acc += ((k.uu >> a) & 1u) ? lv[a] * out[a].t[0] : 0.0;
         if (m < 3 && !((k.uu >> m) & 1u)) acc += lv[m];   // frozen component: v' = v
         lin[m] = acc;
+    }
+    }   // phases 1-2
+    if constexpr (kPart == 1) {
+        if (live) {
+            for (int a = 0; a < 3; ++a) {
+                lp.x[a][i] = lx[a];
+                lp.v[a][i] = lin[a];
+            }
+            for (int m = 0; m < 6; ++m) scratch[m * n + i] = lin[3 + m];
+            scratch[6 * n + i] = 1.0;
+        } else if (i < n) {
+            scratch[6 * n + i] = 0.0;
+        }
+        continue;
+    }
+    if constexpr (kPart == 2) {
+        for (int m = 0; m < 6; ++m) lin[3 + m] = live ? scratch[m * n + i] : 0.0;
     }
     // ---- phase 3: gather stencils, dE_p/dx, scatter box
     AdjGridN<kPts> gr;
@@ -901,7 +925,7 @@ __global__ __launch_bounds__(kAdjThreads, kMinBlocks) void adjoint_particles_ker
         }
         for (int a = 0; a < 3; ++a) {
             lp.x[a][i] = lx[a];
-            lp.v[a][i] = lin[a];
+            if constexpr (kPart != 2) lp.v[a][i] = lin[a];
         }
     }
     __syncthreads();
@@ -1095,7 +1119,7 @@ int pic_adjoint_particles(const PicDesc* desc, const PicSpecies* sp, const PicPa
                           const float* const E[3], const float* const B[3], const float* const E_ext[3],
                           const float* const B_ext[3], const double* const lamJ[3], double* const lam_x[3],
                           double* const lam_v[3], double* const lamE[3], double* const lamB[3], int32_t min_blocks,
-                          int32_t* workspace, void* stream) {
+                          int32_t* workspace, double* scratch, void* stream) {
     PIC_REQUIRE(desc_ok(desc) && periodic_all(desc) && particles_ok(sp, state) && E && B && lamJ && lam_x &&
                 lam_v && lamE && lamB && workspace);
     int64_t n = (int64_t)sp->count * sp->slots;
@@ -1119,28 +1143,25 @@ int pic_adjoint_particles(const PicDesc* desc, const PicSpecies* sp, const PicPa
     hipStream_t st = as_stream(stream);
     if (desc->shape_factor != 1) {
         hipLaunchKernelGGL((adjoint_particles_kernel<false, 1>), grid, blk, 0, st, *desc, *sp, *state, fp, lp, n,
-                           nullptr, nullptr, nullptr, nullptr);
+                           nullptr, nullptr, nullptr, nullptr, nullptr);
         PIC_CHECK_LAUNCH();
         return 0;
     }
+    PIC_REQUIRE(scratch);
     hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, count);
     const bool vjp = desc->pusher == 0 && desc->variant != 1;
-#define PIC_ADJ_LAUNCH(MB, PUSH)                                                                                  \
-    hipLaunchKernelGGL((adjoint_particles_kernel<true, MB, PUSH>), grid, blk, 0, st, *desc, *sp, *state, fp, lp, n, \
-                       list, count, nullptr, nullptr)
-    if (min_blocks >= 8) {
-        if (vjp) PIC_ADJ_LAUNCH(8, 1);
-        else PIC_ADJ_LAUNCH(8, 2);
-    } else if (min_blocks >= 4) {
-        if (vjp) PIC_ADJ_LAUNCH(4, 1);
-        else PIC_ADJ_LAUNCH(4, 2);
-    } else {
-        if (vjp) PIC_ADJ_LAUNCH(1, 1);
-        else PIC_ADJ_LAUNCH(1, 2);
-    }
+    // first-order path: phases 1-2, then phase 3; unforced, each lands at two
+    // waves per SIMD (about 185 VGPRs), where a register target would spill
+    (void)min_blocks;
+#define PIC_ADJ_LAUNCH(PUSH, PART)                                                                                \
+    hipLaunchKernelGGL((adjoint_particles_kernel<true, 1, PUSH, PART>), grid, blk, 0, st, *desc, *sp, *state, fp,  \
+                       lp, n, list, count, nullptr, nullptr, scratch)
+    if (vjp) PIC_ADJ_LAUNCH(1, 1);
+    else PIC_ADJ_LAUNCH(2, 1);
+    PIC_ADJ_LAUNCH(1, 2);
 #undef PIC_ADJ_LAUNCH
     hipLaunchKernelGGL((adjoint_particles_kernel<false, 1>), grid2, blk, 0, st, *desc, *sp, *state, fp, lp, n, nullptr,
-                       nullptr, list, count);
+                       nullptr, list, count, nullptr);
     PIC_CHECK_LAUNCH();
     return 0;
 }
