@@ -45,7 +45,10 @@ def _check_supported(static, dyn):
 
 
 class AdjointRun:
-    """Forward with checkpoints, then reverse sweep with recomputation."""
+    """Forward with checkpoints, then reverse sweep with recomputation.
+    min_blocks is passed through to pic_adjoint_particles, whose first-order
+    kernels choose their own occupancy (the argument stays for callers that
+    set it)."""
 
     def __init__(self, static_parameters, dynamic_parameters, species_config, active, external_fields=None,
                  checkpoint_every=20, min_blocks=8):
