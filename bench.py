@@ -46,7 +46,7 @@ def measured_traffic(kernel, n, ppc):
     return None, None
 
 
-def cpu_baseline(n=32, ppc=16):
+def cpu_baseline(n=32, ppc=40):
     """FP64 NumPy oracle, one electrodynamic Esirkepov step of an n^3 x ppc warm plasma."""
     import numpy as np
     from oracle import pic_oracle as O
