@@ -29,9 +29,26 @@ Parity pins (checks that do not retype the formula; tests/test_oracle_*.py):
   half-step factor dt/2 that first_order_yee.py:116 applies.
 * CG: single Fourier mode sin(x+y+z) (electrostatic_yee_test.py:92-119) and a
   scipy.sparse direct solve of the same 7-point system.
+* centred gradient: exact cos(theta) sin(d)/d of a single mode per axis, and
+  the half difference on conducting wall planes of a linear phi; the rolled
+  gradient of calculate_electrostatic_fields wraps the interior instead
+  (a(n-2)/2 on the wall planes of a linear phi).
+* zero / constant boundaries and the phi boundary sequence: the known answers
+  of ghost_cells_test.py:97-160 and electrostatic_yee_test.py:150-222, plus an
+  element-wise statement of the result with two conducting axes (only the
+  last conducting axis keeps copied ghosts).
+* J_from_rhov face weights: the second (CIC) and third (TSC) central moments
+  of the face stencil equal those of the extrapolating weights from the node
+  anchor (J_from_rhov.py:139-141), not of a bracketing stencil.
+* Esirkepov reduced-axis currents: totals and the moments x y (2-D null axis,
+  the 1/3-1/6 time average of x(t) y(t)) and x (1-D transverse, the path
+  midpoint).
+* mode="drop": TSC deposits with g = 1 whose stencil passes the upper end
+  (dropped) or index -1 (wrapped to L-1).
 * sort: ``np.argsort(kind="stable")`` is the specification itself
   (particle_initialization.py:159); checked against a brute-force bucket sort.
-* energy: closed forms on uniform fields / single particles.
+* energy, momentum, charge: closed forms on uniform fields / single
+  particles; the charge of a deposit equals sum q w.
 * PML: profile values at hand-computed indices and the validation errors of
   PML.py:17-81; the ADE memory under a constant derivative follows the closed
   form m_n = (b^n - 1) d, so the stretched derivative decays as b^n d;
@@ -886,6 +903,34 @@ def electrostatic_fields(x, active, sp: Species, rho, phi, p: Params, tol=1e-12,
     return E, phi, rho
 
 
+def roll_gradient_E(phi, p: Params):
+    """_centered_finite_difference_gradient of -phi on the interior
+    (electrostatic_yee.py:46-60, 197-207): jnp.roll wraps the interior on every
+    axis whatever the field BC; the result is placed in zero ghosted arrays and
+    refreshed with the field codes."""
+    g = p.g
+    A = slice(g, -g)
+    f = -1.0 * np.asarray(phi, dtype=np.float64)[A, A, A]
+    E = []
+    for a in range(3):
+        grad = (np.roll(f, -1, axis=a) - np.roll(f, 1, axis=a)) / (2.0 * p.d[a])
+        full = np.zeros(p.shape)
+        full[A, A, A] = grad
+        E.append(refresh(full, p))
+    return E
+
+
+def electrostatic_fields_from_rho(rho, phi, p: Params, tol=1e-12, max_iter=5000):
+    """calculate_electrostatic_fields (electrostatic_yee.py:159-209): CG on the
+    given rho (no deposit), phi boundaries, digital filter, phi boundaries,
+    rolled centred gradient.  Returns (E, phi, rho)."""
+    phi = solve_poisson_cg(rho, phi, p, tol=tol, max_iter=max_iter)
+    phi = phi_boundaries(phi, p)
+    phi = digital_filter(phi, p.alpha, p.g)
+    phi = phi_boundaries(phi, p)
+    return roll_gradient_E(phi, p), phi, rho
+
+
 # --------------------------------------------------------------------------
 # particle positions and boundaries (particles/particle_tile_communication.py)
 # --------------------------------------------------------------------------
@@ -968,6 +1013,19 @@ def compute_energy(u, active, sp: Species, E, B, p: Params):
     mom2 = np.square(mass * gamma) * v2
     ke = np.sum(act * (np.sqrt(mom2 * p.C ** 2 + mass ** 2 * p.C ** 4) - mass * p.C ** 2))
     return uE, uB, ke
+
+
+def compute_total_momentum(u, active, sp: Species):
+    """compute_total_momentum (utils.py:190-202): sum active |v| mass weight."""
+    v = np.sqrt(sum(np.asarray(u[..., a], dtype=np.float64) ** 2 for a in range(3)))
+    mass = (np.asarray(sp.mass, dtype=np.float64) * np.asarray(sp.weight, dtype=np.float64))[:, None]
+    return float(np.sum(np.asarray(active, dtype=np.float64) * v * mass))
+
+
+def total_charge(rho, p: Params):
+    """Sum of rho over the interior times the cell volume."""
+    g = p.g
+    return float(np.sum(np.asarray(rho, dtype=np.float64)[g:-g, g:-g, g:-g]) * p.d[0] * p.d[1] * p.d[2])
 
 
 def courant_condition(cfl, p: Params):
