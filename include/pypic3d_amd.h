@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define PIC_ABI_VERSION 6
+#define PIC_ABI_VERSION 7
 #define PIC_MAX_SPECIES 16
 
 /* Scalar run description: StaticParameters + DynamicParameters fields the hot
@@ -240,10 +240,16 @@ int pic_permute_particles(const PicParticles* in, PicParticles* out, const int32
 
 /* ---- electrostatics (solvers/electrostatic_yee.py) ---------------------- */
 int64_t pic_cg_workspace_bytes(const PicDesc* desc);
-/* solve_poisson_with_conjugate_gradient (electrostatic_yee.py:71-156) with
- * FP32 vectors and FP64 dot products; stops when r.r <= max(tol, rtol*|r0|)^2
- * or after max_iter iterations.  phi in/out.  Synchronises the host once per
- * 32 iterations to read the convergence flag.  iterations/residual optional. */
+/* solve_poisson_with_conjugate_gradient (electrostatic_yee.py:71-156) as
+ * mixed-precision iterative refinement: phi and the residual
+ * r = rho/eps + lapl(phi) in FP64, each correction solved by CG with FP32
+ * vectors and FP64 dot products.  Stops when the FP64 ||r|| <= max(tol,
+ * rtol*||r0||), when a refinement pass no longer halves ||r||, or after
+ * max_iter CG iterations in total.  phi in/out (its boundaries applied as
+ * _apply_tiled_phi_constant_boundaries does).  Synchronises the host once
+ * per refinement pass and once per 32 iterations to read the convergence
+ * flag.  iterations (total CG iterations) and residual_norm (final FP64
+ * ||r||) optional. */
 int pic_poisson_cg(const PicDesc* desc, const float* rho, float* phi, double tol, double rtol,
                    int32_t max_iter, void* workspace, int64_t workspace_bytes,
                    int32_t* iterations, double* residual_norm, void* stream);
@@ -251,12 +257,23 @@ int pic_poisson_cg(const PicDesc* desc, const float* rho, float* phi, double tol
  * E interior = -centred grad phi, ghosts refreshed (phi must be refreshed). */
 int pic_efield_from_phi(const PicDesc* desc, const float* phi, float* const E[3], void* stream);
 
-/* ---- diagnostics (utils.py:108-187) ------------------------------------ */
+/* ---- diagnostics (utils.py:108-202) ------------------------------------ */
 int64_t pic_energy_workspace_bytes(const PicDesc* desc, const PicSpecies* sp);
-/* out[3] (device, FP64) = electric, magnetic, kinetic energy, accumulated in FP64. */
+/* compute_energy (utils.py:108-187): out[3] (device, FP64) = electric,
+ * magnetic, kinetic energy, accumulated in FP64. */
 int pic_energy(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
                const float* const E[3], const float* const B[3], double* out,
                void* workspace, int64_t workspace_bytes, void* stream);
+/* compute_total_momentum (utils.py:190-202): out[0] (device, FP64) =
+ * sum over active slots of |v| mass weight.  workspace: at least 8192 bytes. */
+int pic_total_momentum(const PicSpecies* sp, const PicParticles* in, double* out, void* workspace,
+                       int64_t workspace_bytes, void* stream);
+/* Total charge: out[0] (device, FP64) = sum over the interior of rho dx dy dz
+ * (the charge diagnostic; the reference sums rho with jnp.sum where its tests
+ * check charge, e.g. single_particle_pipeline_test.py:429-431).  workspace:
+ * at least 8192 bytes. */
+int pic_total_charge(const PicDesc* desc, const float* rho, double* out, void* workspace, int64_t workspace_bytes,
+                     void* stream);
 
 /* ---- batched ensembles ----------------------------------------------------
  * Independent simulations of uneven grid size and particle count advance

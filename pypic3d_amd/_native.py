@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libpypic3d_hip.so")
 MAX_SPECIES = 16
 PUSHERS = {"boris": 0, "higuera_cary": 1}
 FUSED_VARIANTS = 6
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 c_int = ctypes.c_int
 c_i32 = ctypes.c_int32
@@ -101,6 +101,8 @@ SIGNATURES = {
     "pic_efield_from_phi": (c_int, [DP, c_vp, P3, c_vp]),
     "pic_energy_workspace_bytes": (c_i64, [DP, SP]),
     "pic_energy": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, c_i64, c_vp]),
+    "pic_total_momentum": (c_int, [SP, PP, c_vp, c_vp, c_i64, c_vp]),
+    "pic_total_charge": (c_int, [DP, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "pic_batch_check": (c_int, [c_vp, c_i32, c_i64, c_i64]),
     "pic_batch_fused_push_esirkepov": (c_int, [c_vp, c_vp, c_i32, c_i32, c_i32, PP, P3, P3, PP, P3, c_vp]),
     "pic_batch_field_step": (c_int, [c_vp, c_i32, P3, P3, P3, P3, P3, c_vp]),

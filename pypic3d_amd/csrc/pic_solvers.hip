@@ -1,15 +1,34 @@
-// Electrostatic Poisson CG with FP32 vectors and FP64 dot products, and the
-// FP64 energy diagnostics.
+// Electrostatic Poisson CG (FP32 iterations inside an FP64 refinement loop)
+// and the FP64 energy diagnostics.
 //
 // CG follows solve_poisson_with_conjugate_gradient (electrostatic_yee.py:71-156):
 // r0 = rho/eps + lapl(phi), p0 = r0, alpha = r.r / p.(-lapl p), phi += alpha p,
-// r -= alpha (-lapl p), beta = r'.r'/r.r, p = r' + beta p, with the phi/p
-// boundary handling of _apply_tiled_phi_constant_boundaries (:26-43).  The
-// scalars live on the device; every kernel returns at once when the device
-// `done` flag is set, so the host can queue iterations in batches and read the
-// flag once per batch.  Only phi's interior feeds later iterations, so its
-// ghosts are applied once at the end (the reference re-applies them every
-// iteration with no effect on the interior).
+// r -= alpha (-lapl p), beta = r'.r'/r.r, p = r' + beta p.  The reference runs
+// it in FP64 to an absolute tolerance of 1e-12 on ||r||.  Here the potential
+// is held in FP64 and the residual r = rho/eps + lapl(phi) is recomputed from
+// it in FP64 (mixed-precision iterative refinement): each outer pass solves
+// -lapl e = r with FP32 CG vectors and FP64 dot products to a relative
+// residual of kInnerRtol, then phi += e in FP64.  The outer loop stops at the
+// reference's test ||r|| <= max(tol, rtol ||r0||) on the FP64 residual, when a
+// pass no longer halves it (the FP64 rounding floor), or after max_iter inner
+// iterations in total.
+//
+// Ghost handling: phi and p are refreshed by _apply_tiled_phi_constant_
+// boundaries (:26-43) before every Laplacian in the reference.  Only the face
+// neighbours of interior cells enter the 7-point Laplacian, and after that
+// sequence a face neighbour across axis a is: the periodic partner (periodic
+// axis; a reduced periodic axis is its own partner), the cell itself (the
+// last conducting axis, whose ghosts copy the adjacent plane) or zero (an
+// earlier conducting axis, whose ghosts the later calls' refresh zeroes).
+// The kernels read neighbours through that map on interior-only arrays.
+// Without a zero-neighbour axis the operator is singular (constants); the
+// mean of the FP64 residual is then removed before each pass (the k = 0
+// mode, a neutralising background), so a deposit whose net charge is zero
+// only up to FP32 rounding still converges; the reference's FP64 CG relies
+// on the net charge being zero to its own rounding.
+// The scalars live on the device; every iteration kernel returns at once when
+// the device `done` flag is set, so the host queues iterations in batches and
+// reads the flag once per batch.
 #include "pic_common.h"
 
 int pic_refresh_fields(const PicDesc* d, float* f0, float* f1, float* f2, int bc_type, hipStream_t s);
@@ -20,20 +39,39 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxPartials = 1024;
 constexpr int kBatch = 32;
+constexpr int kMaxOuter = 30;
+constexpr double kInnerRtol = 1.0e-5;
+
+enum NeighbourMode : int { NB_PERIODIC = 0, NB_SELF = 1, NB_ZERO = 2 };
 
 struct CGScalars {
-    double rr, pAp, alpha, beta, tol2;
+    double rr, pAp, alpha, beta, tol2, rr_true, mean;
     int32_t done, iter, max_iter, pad;
 };
 
-__device__ inline void interior_coords(const PicDesc& d, int64_t t, int& i, int& j, int& k) {
-    k = (int)(t % d.n[2]);
-    int64_t r = t / d.n[2];
-    j = (int)(r % d.n[1]);
-    i = (int)(r / d.n[1]);
-    i += d.g;
-    j += d.g;
-    k += d.g;
+// interior-array geometry and the face-neighbour map
+struct LapMap {
+    int n[3];
+    int mode[3];
+    int singular;   // no zero-neighbour axis: constants span the null space
+    int64_t sx, sy;
+    double i2[3];   // 1/d^2
+};
+
+LapMap lap_map(const PicDesc* d) {
+    LapMap M;
+    int last = -1;
+    for (int a = 0; a < 3; ++a)
+        if (d->bc[a] == 1) last = a;
+    for (int a = 0; a < 3; ++a) {
+        M.n[a] = d->n[a];
+        M.mode[a] = d->bc[a] != 1 ? NB_PERIODIC : (a == last ? NB_SELF : NB_ZERO);
+        M.i2[a] = 1.0 / (d->d[a] * d->d[a]);
+    }
+    M.singular = M.mode[0] != NB_ZERO && M.mode[1] != NB_ZERO && M.mode[2] != NB_ZERO;
+    M.sy = d->n[2];
+    M.sx = (int64_t)d->n[1] * d->n[2];
+    return M;
 }
 
 __device__ inline double block_sum(double v) {
@@ -49,44 +87,111 @@ __device__ inline double block_sum(double v) {
     return out;
 }
 
-// -lapl f at interior cell c (electrostatic_yee.py:98-103), reading f's ghosts.
-__device__ inline float neg_lapl(const float* __restrict__ f, int64_t c, int64_t sx, int64_t sy, float ix2, float iy2,
-                                 float iz2) {
-    float fc = f[c];
-    float l = (f[c + sx] + f[c - sx] - 2.0f * fc) * ix2 + (f[c + sy] + f[c - sy] - 2.0f * fc) * iy2 +
-              (f[c + 1] + f[c - 1] - 2.0f * fc) * iz2;
-    return -l;
+// value of the face neighbour at offset dir (+-1) along an axis of the
+// interior array: index q, extent n, stride st
+template <class T>
+__device__ inline T face_nb(const T* __restrict__ f, int64_t c, int q, int n, int64_t st, int mode, int dir, T fc) {
+    int qq = q + dir;
+    if (qq >= 0 && qq < n) return f[c + dir * st];
+    if (mode == NB_PERIODIC) return f[c + (dir > 0 ? -(int64_t)(n - 1) : (int64_t)(n - 1)) * st];
+    if (mode == NB_SELF) return fc;
+    return (T)0;
 }
 
-struct LapConst {
-    int64_t sx, sy;
-    float ix2, iy2, iz2;
-};
+// lapl f at interior cell (i, j, k) (electrostatic_yee.py:98-103) in T
+template <class T>
+__device__ inline T lapl_at(const T* __restrict__ f, const LapMap& M, int64_t c, int i, int j, int k) {
+    const T fc = f[c];
+    const T two = (T)2;
+    T lx = face_nb(f, c, i, M.n[0], M.sx, M.mode[0], 1, fc) + face_nb(f, c, i, M.n[0], M.sx, M.mode[0], -1, fc) -
+           two * fc;
+    T ly = face_nb(f, c, j, M.n[1], M.sy, M.mode[1], 1, fc) + face_nb(f, c, j, M.n[1], M.sy, M.mode[1], -1, fc) -
+           two * fc;
+    T lz = face_nb(f, c, k, M.n[2], (int64_t)1, M.mode[2], 1, fc) +
+           face_nb(f, c, k, M.n[2], (int64_t)1, M.mode[2], -1, fc) - two * fc;
+    return lx * (T)M.i2[0] + ly * (T)M.i2[1] + lz * (T)M.i2[2];
+}
 
-__global__ void cg_init_kernel(PicDesc d, LapConst L, const float* __restrict__ rho, const float* __restrict__ phi,
-                               float* __restrict__ r, float* __restrict__ p, double* partials, float inv_eps) {
+__device__ inline void split(const LapMap& M, int64_t t, int& i, int& j, int& k) {
+    k = (int)(t % M.n[2]);
+    int64_t r = t / M.n[2];
+    j = (int)(r % M.n[1]);
+    i = (int)(r / M.n[1]);
+}
+
+__device__ inline void interior_coords(const PicDesc& d, int64_t t, int& i, int& j, int& k) {
+    k = (int)(t % d.n[2]);
+    int64_t r = t / d.n[2];
+    j = (int)(r % d.n[1]);
+    i = (int)(r / d.n[1]);
+    i += d.g;
+    j += d.g;
+    k += d.g;
+}
+
+// phi_d <- interior of phi (ghosted FP32)
+__global__ void cg_load_kernel(PicDesc d, const float* __restrict__ phi, double* __restrict__ phid) {
     int64_t total = (int64_t)d.n[0] * d.n[1] * d.n[2];
-    double acc = 0.0;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k;
         interior_coords(d, t, i, j, k);
-        int64_t c = fidx(d, i, j, k);
-        float rv = rho[c] * inv_eps - neg_lapl(phi, c, L.sx, L.sy, L.ix2, L.iy2, L.iz2);
-        r[c] = rv;
-        p[c] = rv;
-        acc += (double)rv * (double)rv;
+        phid[t] = (double)phi[fidx(d, i, j, k)];
+    }
+}
+
+// sum of the FP64 residual rho/eps + lapl(phi_d) (its mean is removed when
+// the operator is singular)
+__global__ void cg_residual_sum_kernel(PicDesc d, LapMap M, const float* __restrict__ rho,
+                                       const double* __restrict__ phid, double* partials, double inv_eps) {
+    int64_t total = (int64_t)M.n[0] * M.n[1] * M.n[2];
+    double acc = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        split(M, t, i, j, k);
+        acc += (double)rho[fidx(d, i + d.g, j + d.g, k + d.g)] * inv_eps + lapl_at(phid, M, t, i, j, k);
     }
     double s = block_sum(acc);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-__global__ void cg_init_finalize(const double* partials, int nparts, CGScalars* sc, double tol, double rtol,
-                                 int max_iter) {
+__global__ void cg_mean_kernel(const double* partials, int nparts, CGScalars* sc, double inv_n) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) acc += partials[b];
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) sc->mean = acc * inv_n;
+}
+
+// FP64 residual r = rho/eps + lapl(phi_d) - mean; starts an inner solve: r
+// (FP32), p = r, e = 0; partials of r.r in FP64 (of the FP64 residual)
+__global__ void cg_residual_kernel(PicDesc d, LapMap M, const float* __restrict__ rho, const double* __restrict__ phid,
+                                   float* __restrict__ r, float* __restrict__ p, float* __restrict__ e,
+                                   double* partials, double inv_eps, const CGScalars* sc) {
+    int64_t total = (int64_t)M.n[0] * M.n[1] * M.n[2];
+    const double mean = M.singular ? sc->mean : 0.0;
+    double acc = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        split(M, t, i, j, k);
+        double rv = (double)rho[fidx(d, i + d.g, j + d.g, k + d.g)] * inv_eps + lapl_at(phid, M, t, i, j, k) - mean;
+        float rf = (float)rv;
+        r[t] = rf;
+        p[t] = rf;
+        e[t] = 0.0f;
+        acc += rv * rv;
+    }
+    double s = block_sum(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// r.r of the FP64 residual, and the inner solve's start: the inner residual
+// is r cast to FP32, its stop threshold max(target, kInnerRtol ||r||)
+__global__ void cg_start_kernel(const double* partials, int nparts, CGScalars* sc, double target, int max_iter) {
     double acc = 0.0;
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) acc += partials[b];
     double rr = block_sum(acc);
     if (threadIdx.x == 0) {
-        double t = fmax(tol, rtol * sqrt(rr));
+        double t = fmax(target, kInnerRtol * sqrt(rr));
+        sc->rr_true = rr;
         sc->rr = rr;
         sc->tol2 = t * t;
         sc->iter = 0;
@@ -95,18 +200,17 @@ __global__ void cg_init_finalize(const double* partials, int nparts, CGScalars* 
     }
 }
 
-__global__ void cg_ap_kernel(PicDesc d, LapConst L, const float* __restrict__ p, float* __restrict__ Ap,
-                             double* partials, const CGScalars* sc) {
+__global__ void cg_ap_kernel(LapMap M, const float* __restrict__ p, float* __restrict__ Ap, double* partials,
+                             const CGScalars* sc) {
     if (sc->done) return;
-    int64_t total = (int64_t)d.n[0] * d.n[1] * d.n[2];
+    int64_t total = (int64_t)M.n[0] * M.n[1] * M.n[2];
     double acc = 0.0;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k;
-        interior_coords(d, t, i, j, k);
-        int64_t c = fidx(d, i, j, k);
-        float ap = neg_lapl(p, c, L.sx, L.sy, L.ix2, L.iy2, L.iz2);
-        Ap[c] = ap;
-        acc += (double)p[c] * (double)ap;
+        split(M, t, i, j, k);
+        float ap = -lapl_at(p, M, t, i, j, k);
+        Ap[t] = ap;
+        acc += (double)p[t] * (double)ap;
     }
     double s = block_sum(acc);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
@@ -123,19 +227,16 @@ __global__ void cg_alpha_kernel(const double* partials, int nparts, CGScalars* s
     }
 }
 
-__global__ void cg_update_kernel(PicDesc d, float* __restrict__ phi, float* __restrict__ r, const float* __restrict__ p,
-                                 const float* __restrict__ Ap, double* partials, const CGScalars* sc) {
+__global__ void cg_update_kernel(int64_t total, float* __restrict__ e, float* __restrict__ r,
+                                 const float* __restrict__ p, const float* __restrict__ Ap, double* partials,
+                                 const CGScalars* sc) {
     if (sc->done) return;
     float alpha = (float)sc->alpha;
-    int64_t total = (int64_t)d.n[0] * d.n[1] * d.n[2];
     double acc = 0.0;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k;
-        interior_coords(d, t, i, j, k);
-        int64_t c = fidx(d, i, j, k);
-        phi[c] += alpha * p[c];
-        float rn = r[c] - alpha * Ap[c];
-        r[c] = rn;
+        e[t] += alpha * p[t];
+        float rn = r[t] - alpha * Ap[t];
+        r[t] = rn;
         acc += (double)rn * (double)rn;
     }
     double s = block_sum(acc);
@@ -155,15 +256,26 @@ __global__ void cg_beta_kernel(const double* partials, int nparts, CGScalars* sc
     }
 }
 
-__global__ void cg_p_kernel(PicDesc d, const float* __restrict__ r, float* __restrict__ p, const CGScalars* sc) {
+__global__ void cg_p_kernel(int64_t total, const float* __restrict__ r, float* __restrict__ p, const CGScalars* sc) {
     if (sc->done) return;
     float beta = (float)sc->beta;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x)
+        p[t] = r[t] + beta * p[t];
+}
+
+// phi_d += e (FP64 accumulation of the FP32 correction)
+__global__ void cg_accumulate_kernel(int64_t total, double* __restrict__ phid, const float* __restrict__ e) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x)
+        phid[t] += (double)e[t];
+}
+
+// phi interior <- phi_d rounded to FP32
+__global__ void cg_store_kernel(PicDesc d, const double* __restrict__ phid, float* __restrict__ phi) {
     int64_t total = (int64_t)d.n[0] * d.n[1] * d.n[2];
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k;
         interior_coords(d, t, i, j, k);
-        int64_t c = fidx(d, i, j, k);
-        p[c] = r[c] + beta * p[c];
+        phi[fidx(d, i, j, k)] = (float)phid[t];
     }
 }
 
@@ -227,6 +339,39 @@ __global__ void energy_finalize(const double* fpart, int nf, const double* kpart
     }
 }
 
+// compute_total_momentum (utils.py:190-202): sum active |v| mass weight
+__global__ void momentum_kernel(PicSpecies sp, PicParticles in, double* partials, int64_t n) {
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (!in.active[i]) continue;
+        int s = species_of(i, sp.slots, sp.count);
+        double vx = in.u[0][i], vy = in.u[1][i], vz = in.u[2][i];
+        acc += sqrt(vx * vx + vy * vy + vz * vz) * (sp.mass[s] * sp.weight[s]);
+    }
+    double s = block_sum(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// sum of rho over the interior (the total charge is this times dV)
+__global__ void charge_kernel(PicDesc d, const float* __restrict__ rho, double* partials) {
+    int64_t total = (int64_t)d.n[0] * d.n[1] * d.n[2];
+    double acc = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        interior_coords(d, t, i, j, k);
+        acc += (double)rho[fidx(d, i, j, k)];
+    }
+    double s = block_sum(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void sum_finalize(const double* partials, int n, double scale, double* out) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partials[i];
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) out[0] = scale * acc;
+}
+
 int partial_blocks(int64_t work) {
     int64_t b = (work + kBlock - 1) / kBlock;
     if (b < 1) b = 1;
@@ -256,19 +401,31 @@ int phi_bc(const PicDesc* d, float* f, hipStream_t s) {
 }
 
 struct CGLayout {
-    int64_t field_bytes, p_off, r_off, ap_off, part_off, sc_off, total;
+    int64_t phid_off, p_off, r_off, ap_off, e_off, part_off, sc_off, total;
 };
 
+// workspace: phi_d (FP64) and p, r, Ap, e (FP32) over the interior, FP64
+// partial sums, the scalar block
 CGLayout cg_layout(const PicDesc* d) {
+    auto up = [](int64_t b) { return ((b + 255) / 256) * 256; };
+    const int64_t n = (int64_t)d->n[0] * d->n[1] * d->n[2];
     CGLayout L;
-    L.field_bytes = ((field_size(*d) * 4 + 255) / 256) * 256;
-    L.p_off = 0;
-    L.r_off = L.field_bytes;
-    L.ap_off = 2 * L.field_bytes;
-    L.part_off = 3 * L.field_bytes;
+    L.phid_off = 0;
+    L.p_off = up(n * 8);
+    L.r_off = L.p_off + up(n * 4);
+    L.ap_off = L.r_off + up(n * 4);
+    L.e_off = L.ap_off + up(n * 4);
+    L.part_off = L.e_off + up(n * 4);
     L.sc_off = L.part_off + kMaxPartials * 8;
     L.total = L.sc_off + 256;
     return L;
+}
+
+int read_scalars(CGScalars* host, const CGScalars* dev, hipStream_t s) {
+    hipError_t e = hipMemcpyAsync(host, dev, sizeof(CGScalars), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return (int)e;
+    e = hipStreamSynchronize(s);
+    return e == hipSuccess ? 0 : (int)e;
 }
 
 }  // namespace
@@ -284,64 +441,78 @@ int pic_poisson_cg(const PicDesc* desc, const float* rho, float* phi, double tol
                    void* workspace, int64_t workspace_bytes, int32_t* iterations, double* residual_norm,
                    void* stream) {
     PIC_REQUIRE(desc_ok(desc) && rho && phi && workspace && rho != phi && max_iter >= 0);
+    PIC_REQUIRE(tol >= 0.0 && rtol >= 0.0);
     CGLayout Lw = cg_layout(desc);
     PIC_REQUIRE(workspace_bytes >= Lw.total);
     hipStream_t s = as_stream(stream);
     char* base = static_cast<char*>(workspace);
+    double* phid = reinterpret_cast<double*>(base + Lw.phid_off);
     float* p = reinterpret_cast<float*>(base + Lw.p_off);
     float* r = reinterpret_cast<float*>(base + Lw.r_off);
     float* Ap = reinterpret_cast<float*>(base + Lw.ap_off);
+    float* e = reinterpret_cast<float*>(base + Lw.e_off);
     double* partials = reinterpret_cast<double*>(base + Lw.part_off);
     CGScalars* sc = reinterpret_cast<CGScalars*>(base + Lw.sc_off);
 
-    LapConst L;
-    L.sx = (int64_t)desc->dims[1] * desc->dims[2];
-    L.sy = desc->dims[2];
-    L.ix2 = (float)(1.0 / (desc->d[0] * desc->d[0]));
-    L.iy2 = (float)(1.0 / (desc->d[1] * desc->d[1]));
-    L.iz2 = (float)(1.0 / (desc->d[2] * desc->d[2]));
-    int64_t interior = (int64_t)desc->n[0] * desc->n[1] * desc->n[2];
-    int nb = partial_blocks(interior);
+    const LapMap M = lap_map(desc);
+    const int64_t interior = (int64_t)desc->n[0] * desc->n[1] * desc->n[2];
+    const int nb = partial_blocks(interior);
+    const double inv_eps = 1.0 / desc->eps;
 
-    // r, p ghosts must hold defined values (p's are rewritten by phi_bc)
-    int rc = pic_fill_f32(p, field_size(*desc), 0.0f, stream);
+    int rc = phi_bc(desc, phi, s);          // phi = apply_bc(phi) (:139)
     if (rc) return rc;
-    rc = phi_bc(desc, phi, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(cg_init_kernel, dim3(nb), dim3(kBlock), 0, s, *desc, L, rho, phi, r, p, partials,
-                       (float)(1.0 / desc->eps));
+    hipLaunchKernelGGL(cg_load_kernel, dim3(nb), dim3(kBlock), 0, s, *desc, phi, phid);
     PIC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(cg_init_finalize, dim3(1), dim3(kBlock), 0, s, partials, nb, sc, tol, rtol, (int)max_iter);
-    PIC_CHECK_LAUNCH();
-    rc = phi_bc(desc, p, s);
-    if (rc) return rc;
 
     CGScalars host;
-    for (int it = 0; it < max_iter + kBatch; it += kBatch) {
-        hipError_t e = hipMemcpyAsync(&host, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, s);
-        if (e != hipSuccess) return (int)e;
-        e = hipStreamSynchronize(s);
-        if (e != hipSuccess) return (int)e;
-        if (host.done) break;
-        for (int b = 0; b < kBatch; ++b) {
-            hipLaunchKernelGGL(cg_ap_kernel, dim3(nb), dim3(kBlock), 0, s, *desc, L, p, Ap, partials, sc);
-            hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kBlock), 0, s, partials, nb, sc);
-            hipLaunchKernelGGL(cg_update_kernel, dim3(nb), dim3(kBlock), 0, s, *desc, phi, r, p, Ap, partials, sc);
-            hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(kBlock), 0, s, partials, nb, sc);
-            hipLaunchKernelGGL(cg_p_kernel, dim3(nb), dim3(kBlock), 0, s, *desc, r, p, sc);
-            PIC_CHECK_LAUNCH();
-            rc = phi_bc(desc, p, s);
-            if (rc) return rc;
+    int total_iter = 0;
+    double target = tol, prev = 0.0, last = 0.0;
+    for (int outer = 0; outer < kMaxOuter; ++outer) {
+        if (M.singular) {
+            hipLaunchKernelGGL(cg_residual_sum_kernel, dim3(nb), dim3(kBlock), 0, s, *desc, M, rho, phid, partials,
+                               inv_eps);
+            hipLaunchKernelGGL(cg_mean_kernel, dim3(1), dim3(kBlock), 0, s, partials, nb, sc, 1.0 / (double)interior);
         }
+        hipLaunchKernelGGL(cg_residual_kernel, dim3(nb), dim3(kBlock), 0, s, *desc, M, rho, phid, r, p, e, partials,
+                           inv_eps, sc);
+        hipLaunchKernelGGL(cg_start_kernel, dim3(1), dim3(kBlock), 0, s, partials, nb, sc, target,
+                           (int)(max_iter - total_iter));
+        PIC_CHECK_LAUNCH();
+        rc = read_scalars(&host, sc, s);
+        if (rc) return rc;
+        last = sqrt(host.rr_true);
+        if (outer == 0) target = fmax(tol, rtol * last);
+        if (!(last > target) || total_iter >= max_iter) break;
+        if (outer > 0 && !(last < 0.5 * prev)) break;    // at the FP64 rounding floor
+        prev = last;
+        // inner FP32 CG on -lapl e = r, e0 = 0, to max(target, kInnerRtol ||r||)
+        hipLaunchKernelGGL(cg_start_kernel, dim3(1), dim3(kBlock), 0, s, partials, nb, sc, target,
+                           (int)(max_iter - total_iter));
+        PIC_CHECK_LAUNCH();
+        for (;;) {
+            for (int b = 0; b < kBatch; ++b) {
+                hipLaunchKernelGGL(cg_ap_kernel, dim3(nb), dim3(kBlock), 0, s, M, p, Ap, partials, sc);
+                hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kBlock), 0, s, partials, nb, sc);
+                hipLaunchKernelGGL(cg_update_kernel, dim3(nb), dim3(kBlock), 0, s, interior, e, r, p, Ap, partials,
+                                   sc);
+                hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(kBlock), 0, s, partials, nb, sc);
+                hipLaunchKernelGGL(cg_p_kernel, dim3(nb), dim3(kBlock), 0, s, interior, r, p, sc);
+                PIC_CHECK_LAUNCH();
+            }
+            rc = read_scalars(&host, sc, s);
+            if (rc) return rc;
+            if (host.done) break;
+        }
+        total_iter += host.iter;
+        hipLaunchKernelGGL(cg_accumulate_kernel, dim3(nb), dim3(kBlock), 0, s, interior, phid, e);
+        PIC_CHECK_LAUNCH();
     }
+    hipLaunchKernelGGL(cg_store_kernel, dim3(nb), dim3(kBlock), 0, s, *desc, phid, phi);
+    PIC_CHECK_LAUNCH();
     rc = phi_bc(desc, phi, s);
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(&host, sc, sizeof(CGScalars), hipMemcpyDeviceToHost, s);
-    if (e != hipSuccess) return (int)e;
-    e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return (int)e;
-    if (iterations) *iterations = host.iter;
-    if (residual_norm) *residual_norm = sqrt(host.rr);
+    if (iterations) *iterations = total_iter;
+    if (residual_norm) *residual_norm = last;
     return 0;
 }
 
@@ -371,6 +542,39 @@ int pic_energy(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in
     double dV = desc->d[0] * desc->d[1] * desc->d[2];
     hipLaunchKernelGGL(energy_finalize, dim3(1), dim3(kBlock), 0, s, fpart, nf, kpart, nk, 0.5 * desc->eps * dV,
                        0.5 / desc->mu * dV, out);
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+/* compute_total_momentum (utils.py:190-202), FP64 accumulation. */
+int pic_total_momentum(const PicSpecies* sp, const PicParticles* in, double* out, void* workspace,
+                       int64_t workspace_bytes, void* stream) {
+    PIC_REQUIRE(sp && in && out && workspace);
+    PIC_REQUIRE(particles_ok(sp, in));
+    PIC_REQUIRE(workspace_bytes >= (int64_t)kMaxPartials * 8);
+    hipStream_t s = as_stream(stream);
+    double* part = static_cast<double*>(workspace);
+    int64_t n = (int64_t)sp->count * sp->slots;
+    int nk = partial_blocks(n);
+    hipLaunchKernelGGL(momentum_kernel, dim3(nk), dim3(kBlock), 0, s, *sp, *in, part, n);
+    PIC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(sum_finalize, dim3(1), dim3(kBlock), 0, s, part, nk, 1.0, out);
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+/* Total charge sum_interior rho dV, FP64 accumulation of the FP32 density. */
+int pic_total_charge(const PicDesc* desc, const float* rho, double* out, void* workspace, int64_t workspace_bytes,
+                     void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && rho && out && workspace);
+    PIC_REQUIRE(workspace_bytes >= (int64_t)kMaxPartials * 8);
+    hipStream_t s = as_stream(stream);
+    double* part = static_cast<double*>(workspace);
+    int nf = partial_blocks((int64_t)desc->n[0] * desc->n[1] * desc->n[2]);
+    hipLaunchKernelGGL(charge_kernel, dim3(nf), dim3(kBlock), 0, s, *desc, rho, part);
+    PIC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(sum_finalize, dim3(1), dim3(kBlock), 0, s, part, nf, desc->d[0] * desc->d[1] * desc->d[2],
+                       out);
     PIC_CHECK_LAUNCH();
     return 0;
 }

@@ -1,11 +1,13 @@
 """Electrostatic Poisson CG and field (PyPIC3D/solvers/electrostatic_yee.py:71-276).
 
-The CG keeps FP32 vectors with FP64 dot products and device-resident scalars.
-The reference's default ``tol=1e-12`` is an absolute bound on ||r|| that FP32
-vectors cannot reach when rho is O(1), so the GPU solve stops at
-||r|| <= max(tol, rtol * ||r0||) with ``rtol`` = 1e-6 by default (FP32 floor),
-or after ``max_iter`` iterations.  ``last_solve_info`` holds the iteration
-count and final residual norm of the most recent solve.
+The CG keeps the potential and the residual in FP64 (recomputed from the
+potential, r = rho/eps + lapl(phi)) and solves each correction with FP32 CG
+vectors and FP64 dot products (mixed-precision iterative refinement).  It
+stops at the reference's test ||r|| <= tol (absolute, default 1e-12) on the
+FP64 residual, optionally relaxed to ``rtol * ||r0||``, when a refinement pass
+no longer halves ||r|| (the FP64 rounding floor of a large grid), or after
+``max_iter`` CG iterations in total.  ``last_solve_info`` holds the iteration
+count and final FP64 residual norm of the most recent solve.
 """
 
 from __future__ import annotations
@@ -18,7 +20,7 @@ from .. import _native as N
 from .._tensors import as_field3, check_fields, like_input, make_desc, make_species, particle_planes, pic_particles
 from ..deposition.rho import deposit_rho_into
 
-DEFAULT_RTOL = 1.0e-6
+DEFAULT_RTOL = 0.0
 last_solve_info = {"iterations": None, "residual_norm": None}
 _WORKSPACE: dict = {}
 
@@ -98,3 +100,39 @@ def calculate_tiled_electrostatic_fields(static_parameters, dynamic_parameters, 
                               N.stream_handle(), tol=tol, max_iter=max_iter, rtol=rtol)
     E = tuple(like_input(e, rho_tiles) for e in E3)
     return E, like_input(phi3, phi_tiles), like_input(rho3, rho_tiles)
+
+
+def calculate_electrostatic_fields(static_parameters, dynamic_parameters, particles, rho, phi, solver, bc,
+                                   tol=1e-12, max_iter=5000, rtol=DEFAULT_RTOL):
+    """Single-tile electrostatic field solve from a given rho
+    (electrostatic_yee.py:159-209): CG -> phi boundaries -> digital filter ->
+    phi boundaries -> E = -centred gradient of the interior with periodic
+    wrap (jnp.roll on the interior, :46-60) -> E ghosts refreshed with the
+    field codes.  rho is not deposited here (particles, solver and bc are
+    accepted and unused, as in the reference).  Returns ((Ex, Ey, Ez), phi, rho)."""
+    del particles, solver, bc
+    desc = make_desc(static_parameters, dynamic_parameters)
+    rho3 = as_field3(rho, "rho")
+    phi3 = as_field3(phi, "phi").clone()
+    check_fields(desc, rho3, phi3)
+    s = N.stream_handle()
+    poisson_cg_into(desc, rho3, phi3, tol, max_iter, s, rtol=rtol)
+    phi_boundaries_into(desc, phi3, s)
+    if desc.alpha != 1.0:
+        tmp = phi3.clone()
+        N.call("pic_filter_digital", desc, N.ptr(phi3), N.ptr(tmp), float(desc.alpha), s)
+        phi3.copy_(tmp)
+    phi_boundaries_into(desc, phi3, s)
+    # the roll wraps the interior on every axis: a periodic refresh of a copy
+    # supplies exactly those neighbours
+    periodic = N.PicDesc.from_buffer_copy(desc)
+    for a in range(3):
+        periodic.bc[a] = 0
+    wrapped = phi3.clone()
+    N.call("pic_halo_refresh", periodic, N.ptr(wrapped), 0, s)
+    E3 = [torch.zeros_like(phi3) for _ in range(3)]
+    N.call("pic_efield_from_phi", periodic, N.ptr(wrapped), N.p3(E3), s)
+    for c in E3:
+        N.call("pic_halo_refresh", desc, N.ptr(c), 0, s)
+    E = tuple(like_input(e, phi) for e in E3)
+    return E, like_input(phi3, phi), rho

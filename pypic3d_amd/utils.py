@@ -1,4 +1,5 @@
 """Hot-path helpers from PyPIC3D/utils.py: energy diagnostic (:108-187),
+momentum diagnostic (:190-202), total charge,
 external-field sum (:205-216) and the Courant time step (:761-801).
 
 ``compute_energy`` accumulates in FP64 on the GPU and returns three 0-d
@@ -69,3 +70,44 @@ def is_all_zero(t) -> bool:
         _ZERO_CACHE.clear()
     _ZERO_CACHE[id(t)] = (weakref.ref(t), t._version, res)
     return res
+
+
+_DIAG_WS: dict = {}
+_DIAG_BYTES = 8192
+
+
+def _diag_workspace(dev):
+    ws = _DIAG_WS.get(str(dev))
+    if ws is None:
+        ws = torch.empty(_DIAG_BYTES, dtype=torch.uint8, device=dev)
+        _DIAG_WS[str(dev)] = ws
+    return ws
+
+
+def compute_total_momentum(particles, species_config=None):
+    """Scalar momentum diagnostic sum(active |v| mass weight) (utils.py:190-202),
+    accumulated in FP64 on the GPU; a 0-d float64 CUDA tensor."""
+    if species_config is None:
+        raise ValueError("species_config is required for the momentum")
+    xp, up, act = particle_planes(particles)
+    sp = make_species(species_config, xp[0].shape[1])
+    dev = xp[0].device
+    ws = _diag_workspace(dev)
+    out = torch.empty(1, dtype=torch.float64, device=dev)
+    N.call("pic_total_momentum", sp, pic_particles(xp, up, act), N.ptr(out), N.ptr(ws), ws.numel(),
+           N.stream_handle())
+    return out[0]
+
+
+def compute_total_charge(rho, static_parameters, dynamic_parameters):
+    """Total charge sum_interior(rho) dx dy dz, accumulated in FP64 on the GPU
+    (the charge diagnostic kept beside compute_energy); a 0-d float64 CUDA
+    tensor.  The reference has no function of this name: its tests sum rho
+    the same way (single_particle_pipeline_test.py:429-431)."""
+    desc = make_desc(static_parameters, dynamic_parameters)
+    rho3 = as_field3(rho, "rho")
+    check_fields(desc, rho3)
+    ws = _diag_workspace(rho3.device)
+    out = torch.empty(1, dtype=torch.float64, device=rho3.device)
+    N.call("pic_total_charge", desc, N.ptr(rho3), N.ptr(out), N.ptr(ws), ws.numel(), N.stream_handle())
+    return out[0]

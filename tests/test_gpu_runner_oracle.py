@@ -1,0 +1,219 @@
+"""The benchmarked kernels against the FP64 oracle.
+
+* The tile runner on an exact grid (the fused tile Esirkepov kernel with its
+  rare-list kernel, or the fused tile direct-deposit kernel with each current
+  filter), re-sorting as it runs: every step is compared with one oracle step
+  taken from the runner's own state before the step (the single-step 1e-5
+  bar), and a free run of the oracle from the initial state is compared with
+  the runner after all steps through the composed sort permutations.
+* The Weibel configuration (two anisotropic electron populations) on an 8^3
+  grid for 100 steps, each step checked against the oracle.
+* The 256^3 x 32 ppc warm plasma and the 128^3 x 2 x 16 ppc Weibel run at full
+  size, through size-independent properties every step: discrete continuity,
+  total current, FP64 energy, and the length of the rare list.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import pic_oracle as O
+from pypic3d_amd import _native as N
+from pypic3d_amd import workloads
+from pypic3d_amd.deposition.rho import deposit_rho_into
+from pypic3d_amd.runner import ElectrodynamicRunner
+from pypic3d_amd.utils import compute_energy
+from tests.gpu_helpers import TOL, gpu_field, host, host_particles, oracle_params, oracle_species, rel_err
+from tests.helpers import random_field
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(static, dyn, p, ext=True, seed=70):
+    fields = workloads.empty_fields(static, dyn)
+    E0 = [random_field(p, seed + c, scale=0.05) for c in range(3)]
+    B0 = [random_field(p, seed + 3 + c, scale=0.05) for c in range(3)]
+    z = [np.zeros(p.shape)] * 3
+    Ee = [random_field(p, seed + 6 + c, scale=0.02) for c in range(3)] if ext else z
+    Be = [random_field(p, seed + 9 + c, scale=0.02) for c in range(3)] if ext else z
+    g = lambda a: tuple(gpu_field(c) for c in a)   # noqa: E731
+    gpu = (g(E0), g(B0)) + fields[2:5] + ((g(Ee), g(Be)),) + fields[6:]
+    return gpu, (Ee, Be) if ext else (None, None)
+
+
+def _snapshot(r):
+    parts, fields = r.state()
+    x, u, a = host_particles(parts)
+    return x, u, a, [host(c) for c in fields[0]], [host(c) for c in fields[1]], [host(c) for c in fields[2]]
+
+
+def _compare(got, want, p, what):
+    x, u, a, E, B, J = got
+    np.testing.assert_array_equal(a, want.active, err_msg=what)
+    assert np.max(np.abs(x - want.x)) <= 1e-6 * max(p.wind), what
+    errs = {"u": rel_err(u, want.u)}
+    for c in range(3):
+        errs[f"J{c}"] = rel_err(J[c], want.J[c])
+        errs[f"E{c}"] = rel_err(E[c], want.E[c])
+        errs[f"B{c}"] = rel_err(B[c], want.B[c])
+    return errs
+
+
+def _run_against_oracle(static, dyn, specs, steps, sort_at, seed=3, ext=True, label=""):
+    p = oracle_params(static, dyn)
+    parts, cfg = workloads.plasma(static, dyn, specs, seed=seed)
+    sp = oracle_species(cfg)
+    fields, (Ee, Be) = _fields(static, dyn, p, ext=ext)
+    r = ElectrodynamicRunner(parts, cfg, fields, static, dyn, sort_every=0, tile=None)
+    r.set_tile(4)
+    cum = r.perm.cpu().numpy().astype(np.int64)
+    x0, u0, a0 = host_particles(parts)
+    flat = lambda v: v.reshape(-1, *v.shape[2:])      # noqa: E731
+    free = O.State(x0, u0, a0, [host(c) for c in fields[0]], [host(c) for c in fields[1]], E_ext=Ee, B_ext=Be)
+    worst, rare = {}, []
+    for step in range(steps):
+        x, u, a, E, B, _ = _snapshot(r)
+        want = O.time_loop_electrodynamic(O.State(x, u, a, E, B, E_ext=Ee, B_ext=Be), sp, p)
+        r.tile_ws[0] = 0
+        r.step()
+        if r.tile:
+            rare.append(int(r.tile_ws[0]))
+        errs = _compare(_snapshot(r), want, p, f"{label} step {step}")
+        for k, v in errs.items():
+            worst[k] = max(worst.get(k, 0.0), v)
+        assert max(errs.values()) <= TOL, (label, step, errs)
+        free = O.time_loop_electrodynamic(free, sp, p)
+        if step in sort_at:
+            r.sort()
+            cum = cum[r.perm.cpu().numpy().astype(np.int64)]
+    # free run: the runner's slot i holds the initial slot cum[i]
+    x, u, a, E, B, J = _snapshot(r)
+    S, P = a.shape
+    xf, uf, af = flat(free.x)[cum], flat(free.u)[cum], flat(free.active)[cum]
+    np.testing.assert_array_equal(flat(a), af)
+    drift = {"x": float(np.max(np.abs(flat(x) - xf))), "u": rel_err(flat(u), uf)}
+    for c in range(3):
+        drift[f"J{c}"] = rel_err(J[c], free.J[c])
+        drift[f"E{c}"] = rel_err(E[c], free.E[c])
+        drift[f"B{c}"] = rel_err(B[c], free.B[c])
+    print(f"\n[{label}] worst single-step errors {worst}\n[{label}] free-run drift after {steps} steps {drift}"
+          f"\n[{label}] rare list per step {rare}", flush=True)
+    return worst, drift, rare
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("dep,filt,alpha", [("esirkepov", "none", 1.0), ("direct", "none", 1.0),
+                                            ("direct", "bilinear", 1.0), ("direct", "digital", 0.6)])
+def test_tile_runner_steps_match_oracle(dep, filt, alpha):
+    """16 x 12 x 8 unit cells (an exact grid), a warm electron population with
+    fast tails that leave their tiles and wrap through the periodic faces, a
+    heavy positive species, external fields; 8 steps with re-sorts after steps
+    2 and 5."""
+    static, dyn = workloads.make_parameters((16, 12, 8), cfl=0.5, current_deposition=dep, current_filter=filt,
+                                            alpha=alpha)
+    specs = [workloads.SpeciesSpec(24, weight=0.01, vth=(0.15,) * 3),
+             workloads.SpeciesSpec(6, charge=1.0, mass=20.0, weight=0.04, vth=(0.03,) * 3)]
+    worst, drift, rare = _run_against_oracle(static, dyn, specs, 8, (2, 5), label=f"{dep}/{filt}")
+    assert sum(rare) > 0                         # the rare-list kernel took part
+    # the free FP32 run stays within a few single-step errors of the FP64 run over 8 steps
+    assert drift["x"] <= 1e-5 * max(16, 12, 8)
+    assert max(v for k, v in drift.items() if k != "x") <= 1e-4, drift
+
+
+@pytest.mark.timeout(600)
+def test_weibel_every_step_matches_oracle_for_100_steps():
+    """The Weibel configuration (two electron populations, vth_x 0.1c,
+    vth_y = vth_z 0.01c, 16 ppc each, omega_p dt = 0.1) on 8^3 cells, through
+    the benchmark's tile runner (re-sorting every 10 steps), each of 100 steps
+    checked against one FP64 oracle step from the runner's state."""
+    static, dyn, specs = workloads.config_weibel_3d(n=8, ppc=16)
+    worst, drift, rare = _run_against_oracle(static, dyn, specs, 100, tuple(range(9, 100, 10)), seed=17,
+                                             ext=False, label="weibel 8^3")
+    assert max(worst.values()) <= TOL
+
+
+def _continuity(r, rho_old, rho_new, dt, ppc, w, N):
+    A = (slice(2, -2),) * 3
+    K = slice(1, -3)
+    drho = (rho_new[A] - rho_old[A]).double() / dt
+    Jd = [j.double() for j in r.J]
+    div = (Jd[0][A] - Jd[0][K, 2:-2, 2:-2]) + (Jd[1][A] - Jd[1][2:-2, K, 2:-2]) + (Jd[2][A] - Jd[2][2:-2, 2:-2, K])
+    resid = float((drho + div).abs().max())
+    ulp_max = float(np.spacing(np.float32(max(N) / 2)))
+    bound = ppc * 8 * w * ulp_max / dt
+    return resid, bound
+
+
+def _properties_run(static, dyn, specs, steps, sort_every, label):
+    """Run the tile runner and check, every step: discrete continuity
+    (rho_new - rho_old)/dt + div J within the FP32 rounding of the stored
+    positions, sum J dV = sum q w v, finite FP64 energies with a bounded change
+    of the total, and a rare list below 1 % of the particles."""
+    parts, cfg = workloads.plasma(static, dyn, specs, seed=20261202)
+    r = ElectrodynamicRunner(parts, cfg, workloads.empty_fields(static, dyn), static, dyn, sort_every=sort_every,
+                             tile=4)
+    del parts
+    torch.cuda.empty_cache()
+    N_ = (int(dyn.Nx), int(dyn.Ny), int(dyn.Nz))
+    n_part = r.S * r.P
+    ppc = sum(s.ppc for s in specs)
+    w = max(s.weight for s in specs)
+    qw = torch.tensor(np.asarray(cfg.charge) * np.asarray(cfg.weight), device="cuda", dtype=torch.float64)
+    rho_old = torch.empty_like(r.J[0])
+    rho_new = torch.empty_like(r.J[0])
+    s_ = N.stream_handle()
+
+    def rho_of(out):
+        deposit_rho_into(r.desc, r.sp, r._pp(r.cur), out, "none", s_, tile=r.tile, starts=r.starts)
+
+    def energy():
+        pr, fr = r.state()
+        return [float(v) for v in compute_energy(pr, fr[0], fr[1], static, dyn, cfg)]
+
+    hist = []
+    e_prev = energy()
+    rho_of(rho_old)
+    for step in range(steps):
+        r.tile_ws[0] = 0
+        r.step()
+        rare = int(r.tile_ws[0])
+        rho_of(rho_new)
+        resid, bound = _continuity(r, rho_old, rho_new, float(dyn.dt), ppc, w, N_)
+        assert resid <= bound, (label, step, resid, bound)
+        # total current: sum J dV = sum over active particles of q w v (dV = 1)
+        k = r.cur
+        act = r.act[k].double()
+        for a in range(3):
+            tot = float(((r.vel[k][a].double() * act).sum(dim=1) * qw).sum())
+            scale = float(((r.vel[k][a].double().abs() * act).sum(dim=1) * qw.abs()).sum())
+            got = float(r.J[a][2:-2, 2:-2, 2:-2].double().sum())
+            assert abs(got - tot) <= 1e-5 * scale, (label, step, a, got, tot)
+        e = energy()
+        assert all(np.isfinite(e)), (label, step, e)
+        tot_prev, tot_now = sum(e_prev), sum(e)
+        hist.append((step, e[0], e[1], e[2], rare, resid))
+        assert abs(tot_now - tot_prev) <= 1e-3 * tot_prev, (label, step, e_prev, e)
+        assert rare < 0.01 * n_part, (label, step, rare)
+        e_prev = e
+        rho_old, rho_new = rho_new, rho_old
+    print(f"\n[{label}] (step, U_E, U_B, KE, rare, continuity residual): "
+          + " ".join(f"{h}" for h in hist[:3] + hist[-3:]), flush=True)
+    return hist
+
+
+@pytest.mark.timeout(300)
+def test_warm_plasma_256_cubed_properties():
+    """256^3 x 32 ppc (536,870,912 electrons), the benchmark's workload and
+    runner, 4 steps with a re-sort after step 2."""
+    static, dyn, specs = workloads.config_warm_3d(256, 32)
+    _properties_run(static, dyn, specs, 4, 2, "warm 256^3")
+
+
+@pytest.mark.timeout(300)
+def test_weibel_128_cubed_properties_and_growth():
+    """128^3 x 2 x 16 ppc Weibel for 100 steps (re-sort every 10): the
+    per-step properties, and the magnetic energy grows."""
+    static, dyn, specs = workloads.config_weibel_3d(128, 16)
+    hist = _properties_run(static, dyn, specs, 100, 10, "weibel 128^3")
+    u_b = [h[2] for h in hist]
+    assert u_b[-1] > 2.0 * u_b[9], u_b[::10]
