@@ -237,6 +237,21 @@ __device__ inline void store_vec(float* p, const float v[kP]) {
     else *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
 }
 
+template <int kP>
+__device__ inline void load_vec(const float* p, float v[kP]) {
+    if constexpr (kP == 4) {
+        const float4 q = *reinterpret_cast<const float4*>(p);
+        v[0] = q.x;
+        v[1] = q.y;
+        v[2] = q.z;
+        v[3] = q.w;
+    } else {
+        const float2 q = *reinterpret_cast<const float2*>(p);
+        v[0] = q.x;
+        v[1] = q.y;
+    }
+}
+
 __device__ inline void rare_push(int32_t* rare, int64_t i) {
     const int p = atomicAdd(rare, 1);
     rare[1 + p] = (int32_t)i;
@@ -369,30 +384,63 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
         if (threadIdx.x == 0) sh.q_n = 0;
         __syncthreads();    // also orders the box fills before the first deposits
         const int64_t s0 = base + (int64_t)kP * threadIdx.x;
+        const bool full = vec_ok && s0 >= begin && s0 + kP <= end;
         float acc[12];
 #pragma unroll
         for (int m = 0; m < 12; ++m) acc[m] = 0.0f;
         int h0 = 0, h1 = 0, h2 = 0;
         bool have = false;
+        // The thread's kP slots are loaded at once, with one kP-wide vector load
+        // per plane when the group lies inside the range (the lanes of a wave
+        // then read contiguous memory, and all loads share one round trip).
         // One particle per iteration (the loop stays rolled to bound register
-        // pressure).  Outputs shift through OX/OU/oact so that a group lying
-        // inside the range is stored with one kP-wide vector per plane; slots
-        // sent to the rare list are stored as zeros here and rewritten by the
-        // rare kernel.
-        float OX[3][kP], OU[3][kP];
-        uint32_t oact = 0u;
+        // pressure): slot j's inputs are taken from the front of PX/PU/pact,
+        // which shift by one, and its outputs enter at the back, so after kP
+        // iterations PX/PU/oact hold the outputs in slot order for the vector
+        // stores.  Slots sent to the rare list are stored as zeros here and
+        // rewritten by the rare kernel.
+        float PX[3][kP], PU[3][kP];
+        uint32_t pact = 0u, oact = 0u;
+        if (full) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                load_vec<kP>(in.x[a] + s0, PX[a]);
+                load_vec<kP>(in.u[a] + s0, PU[a]);
+            }
+            if constexpr (kP == 4) pact = *reinterpret_cast<const uint32_t*>(in.active + s0);
+            else pact = *reinterpret_cast<const uint16_t*>(in.active + s0);
+        } else {
+#pragma unroll
+            for (int j = 0; j < kP; ++j) {
+                const int64_t i = s0 + j;
+                const bool inr = i >= begin && i < end;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    PX[a][j] = inr ? in.x[a][i] : 0.0f;
+                    PU[a][j] = inr ? in.u[a][i] : 0.0f;
+                }
+                pact |= (inr && in.active[i] != 0 ? 1u : 0u) << (8 * j);
+            }
+        }
 #pragma unroll 1
         for (int j = 0; j < kP; ++j) {
+            const float xf[3] = {PX[0][0], PX[1][0], PX[2][0]};
+            float v[3] = {PU[0][0], PU[1][0], PU[2][0]};
+            const bool act0 = (pact & 0xffu) != 0u;
+            pact >>= 8;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+#pragma unroll
+                for (int q = 0; q + 1 < kP; ++q) {
+                    PX[a][q] = PX[a][q + 1];
+                    PU[a][q] = PU[a][q + 1];
+                }
+            }
             const int64_t i = s0 + j;
             float ox[3] = {0.0f, 0.0f, 0.0f}, ou[3] = {0.0f, 0.0f, 0.0f};
             uint32_t oa = 0u;
             do {
-            if (i < begin || i >= end) break;
-            const bool act0 = in.active[i] != 0;
-            const float xf[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
-            float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
-            loads_together(xf, v);
-            if (!act0) break;
+            if (!act0) break;          // inactive, or outside [begin, end)
             // ---- anchors on both node lines (grid_and_stencil.py:92-166), exact in FP32
             int anc[3], anv[3];
             float dc[3], dv[3], nodec[3];
@@ -533,22 +581,16 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
             } while (false);
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-#pragma unroll
-                for (int q = 0; q + 1 < kP; ++q) {
-                    OX[a][q] = OX[a][q + 1];
-                    OU[a][q] = OU[a][q + 1];
-                }
-                OX[a][kP - 1] = ox[a];
-                OU[a][kP - 1] = ou[a];
+                PX[a][kP - 1] = ox[a];
+                PU[a][kP - 1] = ou[a];
             }
             oact = (oact >> 8) | (oa << (8 * (kP - 1)));
         }
-        const bool full = vec_ok && s0 >= begin && s0 + kP <= end;
         if (full) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                store_vec<kP>(out.x[a] + s0, OX[a]);
-                store_vec<kP>(out.u[a] + s0, OU[a]);
+                store_vec<kP>(out.x[a] + s0, PX[a]);
+                store_vec<kP>(out.u[a] + s0, PU[a]);
             }
             if constexpr (kP == 4) *reinterpret_cast<uint32_t*>(out.active + s0) = oact;
             else *reinterpret_cast<uint16_t*>(out.active + s0) = (uint16_t)oact;
@@ -559,8 +601,8 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
                 if (i < begin || i >= end) continue;
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    out.x[a][i] = OX[a][j];
-                    out.u[a][i] = OU[a][j];
+                    out.x[a][i] = PX[a][j];
+                    out.u[a][i] = PU[a][j];
                 }
                 out.active[i] = (uint8_t)((oact >> (8 * j)) & 0xffu);
             }
@@ -1055,11 +1097,12 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
     ExactGrid eg;
     if (desc->variant != 1 && make_exact_grid(*desc, eg)) {
         PIC_REQUIRE(workspace);
-        // vector stores of a thread's slot group need 16-byte aligned planes
+        // vector loads and stores of a thread's slot group need 16-byte aligned planes
         int vec_ok = 1;
         for (int a = 0; a < 3; ++a)
-            vec_ok &= ((((uintptr_t)out->x[a]) | ((uintptr_t)out->u[a])) & 15u) == 0;
-        vec_ok &= (((uintptr_t)out->active) & 3u) == 0;
+            vec_ok &= ((((uintptr_t)out->x[a]) | ((uintptr_t)out->u[a]) | ((uintptr_t)in->x[a]) |
+                        ((uintptr_t)in->u[a])) & 15u) == 0;
+        vec_ok &= ((((uintptr_t)out->active) | ((uintptr_t)in->active)) & 3u) == 0;
         hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, workspace);
 #define PIC_EXACT_LAUNCH(T, P, MB)                                                                                  \
         hipLaunchKernelGGL((fused_tile_exact_kernel<T, P, MB>), dim3((unsigned)blocks), dim3(kThreads), 0, st,      \

