@@ -264,6 +264,16 @@ int64_t pic_energy_workspace_bytes(const PicDesc* desc, const PicSpecies* sp);
 int pic_energy(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
                const float* const E[3], const float* const B[3], double* out,
                void* workspace, int64_t workspace_bytes, void* stream);
+/* pic_energy without a workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Its partial sums live
+ * in a static 24 KiB device array per GPU (held until the process ends); it
+ * calls no allocator.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU race.  Returns -2 (with a message on stderr) if a call
+ * would need more partial-sum space than the array holds.  The stream must
+ * belong to the current device. */
+int pic_energy_noalloc(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                       const float* const E[3], const float* const B[3], double* out, void* stream);
 /* compute_total_momentum (utils.py:190-202): out[0] (device, FP64) =
  * sum over active slots of |v| mass weight.  workspace: at least 8192 bytes. */
 int pic_total_momentum(const PicSpecies* sp, const PicParticles* in, double* out, void* workspace,

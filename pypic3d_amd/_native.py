@@ -101,6 +101,7 @@ SIGNATURES = {
     "pic_efield_from_phi": (c_int, [DP, c_vp, P3, c_vp]),
     "pic_energy_workspace_bytes": (c_i64, [DP, SP]),
     "pic_energy": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, c_i64, c_vp]),
+    "pic_energy_noalloc": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp]),
     "pic_total_momentum": (c_int, [SP, PP, c_vp, c_vp, c_i64, c_vp]),
     "pic_total_charge": (c_int, [DP, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "pic_batch_check": (c_int, [c_vp, c_i32, c_i64, c_i64]),

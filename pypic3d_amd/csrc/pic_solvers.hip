@@ -31,13 +31,24 @@
 // reads the flag once per batch.
 #include "pic_common.h"
 
+#include <cstdio>
+#include <mutex>
+
 int pic_refresh_fields(const PicDesc* d, float* f0, float* f1, float* f2, int bc_type, hipStream_t s);
 extern "C" int pic_constant_boundary(const PicDesc* desc, float* field, int axis, void* stream);
+
+// Scratch of the allocation-free energy entry point (pic_energy_noalloc):
+// the FP64 partial sums of its three reductions, 3 x 1024 doubles = 24 KiB,
+// held by every GPU the library runs on until the process ends.  Every
+// partial a call reads is written earlier in the same call, so nothing needs
+// clearing; calls that overlap on one GPU share the array and race.
+static __device__ __align__(256) double scratch[3 * 1024];
 
 namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxPartials = 1024;
+static_assert(sizeof(scratch) == 3 * kMaxPartials * sizeof(double), "energy scratch holds its three partial arrays");
 constexpr int kBatch = 32;
 constexpr int kMaxOuter = 30;
 constexpr double kInnerRtol = 1.0e-5;
@@ -421,6 +432,51 @@ CGLayout cg_layout(const PicDesc* d) {
     return L;
 }
 
+// compute_energy's three FP64 reductions with their partial sums in ws
+// (2 * kMaxPartials field partials, then kMaxPartials kinetic partials)
+int energy_launch(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, const float* const E[3],
+                  const float* const B[3], double* out, double* ws, hipStream_t s) {
+    double* fpart = ws;
+    double* kpart = fpart + 2 * kMaxPartials;
+    int64_t interior = (int64_t)desc->n[0] * desc->n[1] * desc->n[2];
+    int nf = partial_blocks(interior);
+    int64_t n = (int64_t)sp->count * sp->slots;
+    int nk = partial_blocks(n);
+    hipLaunchKernelGGL(energy_fields_kernel, dim3(nf), dim3(kBlock), 0, s, *desc, E[0], E[1], E[2], B[0], B[1], B[2],
+                       fpart);
+    PIC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(energy_kinetic_kernel, dim3(nk), dim3(kBlock), 0, s, *sp, *in, desc->C, kpart, n);
+    PIC_CHECK_LAUNCH();
+    double dV = desc->d[0] * desc->d[1] * desc->d[2];
+    hipLaunchKernelGGL(energy_finalize, dim3(1), dim3(kBlock), 0, s, fpart, nf, kpart, nk, 0.5 * desc->eps * dV,
+                       0.5 / desc->mu * dV, out);
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current (each GPU holds its own copy of the array
+// at its own address; the array's name is not a device address in host code).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= kMaxDevices) return -1;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 int read_scalars(CGScalars* host, const CGScalars* dev, hipStream_t s) {
     hipError_t e = hipMemcpyAsync(host, dev, sizeof(CGScalars), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return (int)e;
@@ -527,23 +583,26 @@ int pic_energy(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in
     for (int a = 0; a < 3; ++a) PIC_REQUIRE(E[a] && B[a]);
     PIC_REQUIRE(particles_ok(sp, in));
     PIC_REQUIRE(workspace_bytes >= pic_energy_workspace_bytes(desc, sp));
-    hipStream_t s = as_stream(stream);
-    double* fpart = static_cast<double*>(workspace);
-    double* kpart = fpart + 2 * kMaxPartials;
-    int64_t interior = (int64_t)desc->n[0] * desc->n[1] * desc->n[2];
-    int nf = partial_blocks(interior);
-    int64_t n = (int64_t)sp->count * sp->slots;
-    int nk = partial_blocks(n);
-    hipLaunchKernelGGL(energy_fields_kernel, dim3(nf), dim3(kBlock), 0, s, *desc, E[0], E[1], E[2], B[0], B[1], B[2],
-                       fpart);
-    PIC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(energy_kinetic_kernel, dim3(nk), dim3(kBlock), 0, s, *sp, *in, desc->C, kpart, n);
-    PIC_CHECK_LAUNCH();
-    double dV = desc->d[0] * desc->d[1] * desc->d[2];
-    hipLaunchKernelGGL(energy_finalize, dim3(1), dim3(kBlock), 0, s, fpart, nf, kpart, nk, 0.5 * desc->eps * dV,
-                       0.5 / desc->mu * dV, out);
-    PIC_CHECK_LAUNCH();
-    return 0;
+    return energy_launch(desc, sp, in, E, B, out, static_cast<double*>(workspace), as_stream(stream));
+}
+
+int pic_energy_noalloc(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, const float* const E[3],
+                       const float* const B[3], double* out, void* stream) {
+    PIC_REQUIRE(desc_ok(desc) && sp && in && E && B && out);
+    for (int a = 0; a < 3; ++a) PIC_REQUIRE(E[a] && B[a]);
+    PIC_REQUIRE(particles_ok(sp, in));
+    // the partial sums this call needs; never more than the scratch holds
+    // (the partial count is capped at kMaxPartials per reduction)
+    const int64_t need = pic_energy_workspace_bytes(desc, sp);
+    if (need < 0 || need > (int64_t)sizeof(scratch)) {
+        fprintf(stderr, "pic_energy_noalloc: needs %lld scratch bytes, the static scratch holds %zu\n",
+                (long long)need, sizeof(scratch));
+        return -2;
+    }
+    double* ws = nullptr;
+    int rc = scratch_address(&ws);
+    if (rc) return rc;
+    return energy_launch(desc, sp, in, E, B, out, ws, as_stream(stream));
 }
 
 /* compute_total_momentum (utils.py:190-202), FP64 accumulation. */
