@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define PIC_ABI_VERSION 7
+#define PIC_ABI_VERSION 8
 #define PIC_MAX_SPECIES 16
 
 /* Scalar run description: StaticParameters + DynamicParameters fields the hot
@@ -284,6 +284,44 @@ int pic_total_momentum(const PicSpecies* sp, const PicParticles* in, double* out
  * at least 8192 bytes. */
 int pic_total_charge(const PicDesc* desc, const float* rho, double* out, void* workspace, int64_t workspace_bytes,
                      void* stream);
+
+/* ---- FP64 path ------------------------------------------------------------
+ * The reference runs in FP64 (__main__.py:186-187).  These entry points take
+ * double field components (same layout) and double particle planes and
+ * follow the reference's expressions in FP64, one thread per slot, any
+ * dimensionality and shape order.  Same semantics as the FP32 entry point of
+ * the same name without the suffix; J/rho are raw deposits into
+ * zero-initialised arrays (pic_fill_f64). */
+typedef struct PicParticlesF64 {
+    double* x[3];
+    double* u[3];
+    uint8_t* active;
+} PicParticlesF64;
+int pic_halo_refresh_f64(const PicDesc* desc, double* field, int bc_type, void* stream);
+int pic_halo_fold_f64(const PicDesc* desc, double* field, int bc_type, void* stream);
+int pic_zero_boundary_f64(const PicDesc* desc, double* field, int axis, void* stream);
+int pic_constant_boundary_f64(const PicDesc* desc, double* field, int axis, void* stream);
+int pic_filter_digital_f64(const PicDesc* desc, const double* in, double* out, double alpha, void* stream);
+int pic_filter_bilinear_f64(const PicDesc* desc, const double* in, double* out, void* stream);
+int pic_yee_update_b_f64(const PicDesc* desc, const double* const E[3], const double* const B_in[3],
+                         double* const B_out[3], void* stream);
+int pic_yee_update_e_f64(const PicDesc* desc, const double* const E_in[3], const double* const B[3],
+                         const double* const J[3], double* const E_out[3], void* stream);
+int pic_push_f64(const PicDesc* desc, const PicSpecies* sp, const PicParticlesF64* in, const double* const E[3],
+                 const double* const B[3], const double* const E_ext[3], const double* const B_ext[3],
+                 double* const u_out[3], void* stream);
+int pic_advance_positions_f64(const PicDesc* desc, const PicSpecies* sp, const PicParticlesF64* in, double dt_scale,
+                              double* const x_out[3], void* stream);
+int pic_particle_boundaries_f64(const PicDesc* desc, const PicSpecies* sp, PicParticlesF64* io, void* stream);
+int pic_deposit_esirkepov_f64(const PicDesc* desc, const PicSpecies* sp, const PicParticlesF64* in,
+                              double* const J[3], void* stream);
+int pic_deposit_direct_f64(const PicDesc* desc, const PicSpecies* sp, const PicParticlesF64* in, double* const J[3],
+                           void* stream);
+int pic_deposit_rho_f64(const PicDesc* desc, const PicSpecies* sp, const PicParticlesF64* in, double* rho,
+                        void* stream);
+/* workspace: at least 24576 bytes */
+int pic_energy_f64(const PicDesc* desc, const PicSpecies* sp, const PicParticlesF64* in, const double* const E[3],
+                   const double* const B[3], double* out, void* workspace, int64_t workspace_bytes, void* stream);
 
 /* ---- batched ensembles ----------------------------------------------------
  * Independent simulations of uneven grid size and particle count advance

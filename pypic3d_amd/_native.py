@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libpypic3d_hip.so")
 MAX_SPECIES = 16
 PUSHERS = {"boris": 0, "higuera_cary": 1}
 FUSED_VARIANTS = 6
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 c_int = ctypes.c_int
 c_i32 = ctypes.c_int32
@@ -104,6 +104,21 @@ SIGNATURES = {
     "pic_energy_noalloc": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp]),
     "pic_total_momentum": (c_int, [SP, PP, c_vp, c_vp, c_i64, c_vp]),
     "pic_total_charge": (c_int, [DP, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "pic_halo_refresh_f64": (c_int, [DP, c_vp, c_int, c_vp]),
+    "pic_halo_fold_f64": (c_int, [DP, c_vp, c_int, c_vp]),
+    "pic_zero_boundary_f64": (c_int, [DP, c_vp, c_int, c_vp]),
+    "pic_constant_boundary_f64": (c_int, [DP, c_vp, c_int, c_vp]),
+    "pic_filter_digital_f64": (c_int, [DP, c_vp, c_vp, c_dbl, c_vp]),
+    "pic_filter_bilinear_f64": (c_int, [DP, c_vp, c_vp, c_vp]),
+    "pic_yee_update_b_f64": (c_int, [DP, P3, P3, P3, c_vp]),
+    "pic_yee_update_e_f64": (c_int, [DP, P3, P3, P3, P3, c_vp]),
+    "pic_push_f64": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, P3, c_vp]),
+    "pic_advance_positions_f64": (c_int, [DP, SP, PP, c_dbl, P3, c_vp]),
+    "pic_particle_boundaries_f64": (c_int, [DP, SP, PP, c_vp]),
+    "pic_deposit_esirkepov_f64": (c_int, [DP, SP, PP, P3, c_vp]),
+    "pic_deposit_direct_f64": (c_int, [DP, SP, PP, P3, c_vp]),
+    "pic_deposit_rho_f64": (c_int, [DP, SP, PP, c_vp, c_vp]),
+    "pic_energy_f64": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, c_i64, c_vp]),
     "pic_batch_check": (c_int, [c_vp, c_i32, c_i64, c_i64]),
     "pic_batch_fused_push_esirkepov": (c_int, [c_vp, c_vp, c_i32, c_i32, c_i32, PP, P3, P3, PP, P3, c_vp]),
     "pic_batch_field_step": (c_int, [c_vp, c_i32, P3, P3, P3, P3, P3, c_vp]),
@@ -156,6 +171,17 @@ def check(rc: int, what: str):
 
 def call(name: str, *args):
     check(getattr(lib(), name)(*args), name)
+
+
+def op(name: str, f64: bool) -> str:
+    """Entry point name for FP32 or FP64 (``_f64`` suffix) tensors."""
+    return name + "_f64" if f64 else name
+
+
+def fill(t, value, stream):
+    """Fill a CUDA tensor (float32 or float64) on the stream with a kernel."""
+    f64 = t.dtype == torch.float64
+    call("pic_fill_f64" if f64 else "pic_fill_f32", ptr(t), t.numel(), float(value), stream)
 
 
 def stream_handle():

@@ -143,8 +143,14 @@ def _np(v):
 
 # ---------------------------------------------------------------- fields
 
+def is_f64(t) -> bool:
+    return isinstance(t, torch.Tensor) and t.dtype == torch.float64
+
+
 def as_field3(t, name="field"):
-    """3-D float32 CUDA view of a field component (accepts the (1,1,1,...) tile shape)."""
+    """3-D CUDA view of a field component (accepts the (1,1,1,...) tile
+    shape): float64 stays float64 (the FP64 path), anything else becomes
+    float32."""
     if not isinstance(t, torch.Tensor):
         raise TypeError(f"{name} must be a torch tensor on the GPU")
     if t.dim() == 6:
@@ -155,19 +161,24 @@ def as_field3(t, name="field"):
         raise ValueError(f"{name}: expected a 3-D field, got shape {tuple(t.shape)}")
     if not t.is_cuda:
         raise ValueError(f"{name}: tensor must live on the GPU")
-    if t.dtype != torch.float32 or not t.is_contiguous():
-        t = t.to(torch.float32).contiguous()
+    want = torch.float64 if t.dtype == torch.float64 else torch.float32
+    if t.dtype != want or not t.is_contiguous():
+        t = t.to(want).contiguous()
     return t
 
 
-def check_fields(desc, *fields):
-    """Every field handed to a kernel must have the descriptor's extents."""
+def check_fields(desc, *fields, dtype=None):
+    """Every field handed to a kernel must have the descriptor's extents and
+    one common dtype (float32, or float64 for the FP64 path)."""
     dims = (desc.dims[0], desc.dims[1], desc.dims[2])
     for f in fields:
         if f is None:
             continue
-        if tuple(f.shape) != dims or f.dtype != torch.float32 or not f.is_contiguous():
-            raise ValueError(f"field of shape {tuple(f.shape)} ({f.dtype}) does not match the grid {dims} float32")
+        if dtype is None:
+            dtype = f.dtype
+        if (tuple(f.shape) != dims or f.dtype != dtype or f.dtype not in (torch.float32, torch.float64)
+                or not f.is_contiguous()):
+            raise ValueError(f"field of shape {tuple(f.shape)} ({f.dtype}) does not match the grid {dims} {dtype}")
 
 
 def like_input(out3, template):
@@ -190,7 +201,7 @@ def _planes(t, name):
         t = t[0, 0, 0]
     if t.dim() != 3 or t.shape[-1] != 3:
         raise ValueError(f"particles.{name}: expected (S, P, 3), got {tuple(t.shape)}")
-    if t.dtype != torch.float32:
+    if t.dtype not in (torch.float32, torch.float64):
         t = t.to(torch.float32)
     planes = [t[..., a] for a in range(3)]
     if not all(p.is_contiguous() for p in planes):
@@ -210,6 +221,8 @@ def particle_planes(particles):
     S, P = xp[0].shape
     if tuple(up[0].shape) != (S, P) or tuple(act.shape) != (S, P):
         raise ValueError("particles: x, u and active disagree on (species, slots)")
+    if xp[0].dtype != up[0].dtype:
+        raise ValueError("particles: x and u must share one dtype")
     return xp, up, act
 
 
@@ -222,9 +235,9 @@ def pic_particles(xp, up, act) -> N.PicParticles:
     return pp
 
 
-def new_soa(S, P, device):
+def new_soa(S, P, device, dtype=torch.float32):
     """(3, S, P) storage and its (1, 1, 1, S, P, 3) view."""
-    store = torch.empty((3, S, P), dtype=torch.float32, device=device)
+    store = torch.empty((3, S, P), dtype=dtype, device=device)
     return store, soa_view(store)
 
 
@@ -232,16 +245,17 @@ def soa_view(store):
     return store.permute(1, 2, 0).unsqueeze(0).unsqueeze(0).unsqueeze(0)
 
 
-def tiled_particles_from_arrays(x, u, active, device="cuda"):
-    """Build TiledParticles on the GPU from (S, P, 3) arrays (NumPy or torch)."""
+def tiled_particles_from_arrays(x, u, active, device="cuda", dtype=torch.float32):
+    """Build TiledParticles on the GPU from (S, P, 3) arrays (NumPy or torch);
+    dtype float32, or float64 for the FP64 path."""
     from .particles.particle_class import TiledParticles
     x = torch.as_tensor(np.asarray(x) if not isinstance(x, torch.Tensor) else x)
     u = torch.as_tensor(np.asarray(u) if not isinstance(u, torch.Tensor) else u)
     a = torch.as_tensor(np.asarray(active) if not isinstance(active, torch.Tensor) else active)
     if x.dim() == 6:
         x, u, a = x[0, 0, 0], u[0, 0, 0], a[0, 0, 0]
-    xs = x.permute(2, 0, 1).to(device=device, dtype=torch.float32).contiguous()
-    us = u.permute(2, 0, 1).to(device=device, dtype=torch.float32).contiguous()
+    xs = x.permute(2, 0, 1).to(device=device, dtype=dtype).contiguous()
+    us = u.permute(2, 0, 1).to(device=device, dtype=dtype).contiguous()
     act = a.to(device=device, dtype=torch.bool).contiguous()
     return TiledParticles(x=soa_view(xs), u=soa_view(us), active=act.unsqueeze(0).unsqueeze(0).unsqueeze(0))
 
@@ -250,5 +264,5 @@ def ceil_div(a, b):
     return (a + b - 1) // b
 
 
-__all__ = ["make_desc", "shape_desc", "make_species", "as_field3", "like_input", "new_field",
+__all__ = ["make_desc", "shape_desc", "make_species", "as_field3", "is_f64", "like_input", "new_field",
            "particle_planes", "pic_particles", "new_soa", "soa_view", "tiled_particles_from_arrays"]

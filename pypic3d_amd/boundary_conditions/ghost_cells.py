@@ -11,7 +11,7 @@ from __future__ import annotations
 import torch
 
 from .. import _native as N
-from .._tensors import as_field3, like_input, make_desc
+from .._tensors import as_field3, is_f64, like_input, make_desc
 
 BC_TYPE_FIELD = 0
 BC_TYPE_PARTICLE = 1
@@ -43,14 +43,15 @@ def _apply(kind, field_tiles, static_parameters, bc_type=0, axis=None):
     out = f.clone()
     d = _desc_for(out, static_parameters)
     s = N.stream_handle()
+    f64 = is_f64(out)
     if kind == "refresh":
-        N.call("pic_halo_refresh", d, N.ptr(out), int(bc_type), s)
+        N.call(N.op("pic_halo_refresh", f64), d, N.ptr(out), int(bc_type), s)
     elif kind == "fold":
-        N.call("pic_halo_fold", d, N.ptr(out), int(bc_type), s)
+        N.call(N.op("pic_halo_fold", f64), d, N.ptr(out), int(bc_type), s)
     elif kind == "zero":
-        N.call("pic_zero_boundary", d, N.ptr(out), int(axis), s)
+        N.call(N.op("pic_zero_boundary", f64), d, N.ptr(out), int(axis), s)
     elif kind == "constant":
-        N.call("pic_constant_boundary", d, N.ptr(out), int(axis), s)
+        N.call(N.op("pic_constant_boundary", f64), d, N.ptr(out), int(axis), s)
     return like_input(out, field_tiles)
 
 

@@ -73,14 +73,27 @@ __device__ inline int anchor_of(double pos, double org, double sp, int sf) {
     return (int)t;
 }
 
-// Node coordinate anchor*spacing + origin rounded like the FP64 reference
-// (product and sum rounded separately, no fused multiply-add).
+// hipcc fuses a*b + c into one multiply-add by default; the reference rounds
+// the product and the sum separately.  These helpers switch contraction off
+// for their own expressions so their roundings are the reference's.
+
+// Node coordinate anchor*spacing + origin rounded like the FP64 reference.
 __device__ inline double node_coord(int a, double sp, double org) {
-    return __dadd_rn(__dmul_rn((double)a, sp), org);
+#pragma clang fp contract(off)
+    return (double)a * sp + org;
 }
 
 // x + v*dt rounded like the FP64 reference (update_tiled_particle_positions).
-__device__ inline double step_pos(double x, double v, double dt) { return __dadd_rn(x, __dmul_rn(v, dt)); }
+__device__ inline double step_pos(double x, double v, double dt) {
+#pragma clang fp contract(off)
+    return x + v * dt;
+}
+
+// (x - org) - (a + 1/2) d, the face offset of J_from_rhov.py:139-141.
+__device__ inline double face_offset(double x, double org, int a, double dd) {
+#pragma clang fp contract(off)
+    return (x - org) - (a + 0.5) * dd;
+}
 
 // shapes.py:5-54 -- weights of stencil points (-1, 0, +1) for r = delta / d.
 __device__ inline void shape3(double r, int sf, double w[3]) {

@@ -1,7 +1,8 @@
 // Per-element field operations shared by the one-launch-per-op kernels
 // (pic_fields.hip) and the batched one-block-per-simulation field step
 // (pic_batch.hip).  Each function handles one element id `t` of its index
-// space; callers map threads to ids.
+// space; callers map threads to ids.  The scalar type T is float for the
+// FP32 path and double for the FP64 path.
 #pragma once
 #include "pic_common.h"
 
@@ -30,8 +31,12 @@ __host__ __device__ inline int64_t fold_elems(const PicDesc& d, int axis) {
     return d.n[axis] == 1 ? (int64_t)da * db : (int64_t)d.g * da * db;
 }
 
+__device__ inline void atomic_add_t(float* p, float v) { unsafeAtomicAdd(p, v); }
+__device__ inline void atomic_add_t(double* p, double v) { unsafeAtomicAdd(p, v); }
+
 // ghost_cells.py:142-196: ghost element t (< refresh_elems) of one axis.
-__device__ inline void refresh_elem(const PicDesc& d, float* f, int axis, int mode, int64_t t) {
+template <class T>
+__device__ inline void refresh_elem(const PicDesc& d, T* f, int axis, int mode, int64_t t) {
     int g = d.g, n = d.n[axis];
     int da, db;
     other_dims(d, axis, da, db);
@@ -40,7 +45,7 @@ __device__ inline void refresh_elem(const PicDesc& d, float* f, int axis, int mo
     int a = (int)(r % da);
     int l = (int)(r / da);
     int dst = l < g ? l : n + l;          // upper ghosts n+g .. n+2g-1
-    float v = 0.0f;
+    T v = (T)0;
     if (mode == MODE_REDUCED_COPY) {
         v = f[slab_to_index(d, axis, g, a, b)];
     } else if (mode == MODE_PERIODIC) {
@@ -52,7 +57,8 @@ __device__ inline void refresh_elem(const PicDesc& d, float* f, int axis, int mo
 
 // ghost_cells.py:218-289: fold element t (< fold_elems) of a non-reduced axis.
 // bc: 0 periodic self-exchange, 1 conducting reflection, other: dropped.
-__device__ inline void fold_elem(const PicDesc& d, float* f, int axis, int bc, int64_t t) {
+template <class T>
+__device__ inline void fold_elem(const PicDesc& d, T* f, int axis, int bc, int64_t t) {
     int g = d.g, n = d.n[axis];
     int da, db;
     other_dims(d, axis, da, db);
@@ -62,36 +68,37 @@ __device__ inline void fold_elem(const PicDesc& d, float* f, int axis, int bc, i
     int l = (int)(r / da);
     int64_t lo_i = slab_to_index(d, axis, l, a, b);
     int64_t hi_i = slab_to_index(d, axis, n + g + l, a, b);
-    float lo = f[lo_i];
-    float hi = f[hi_i];
-    f[lo_i] = 0.0f;
-    f[hi_i] = 0.0f;
+    T lo = f[lo_i];
+    T hi = f[hi_i];
+    f[lo_i] = (T)0;
+    f[hi_i] = (T)0;
     if (bc == 0) {
-        atomic_add_f32(&f[slab_to_index(d, axis, n + l, a, b)], lo);
-        atomic_add_f32(&f[slab_to_index(d, axis, g + l, a, b)], hi);
+        atomic_add_t(&f[slab_to_index(d, axis, n + l, a, b)], lo);
+        atomic_add_t(&f[slab_to_index(d, axis, g + l, a, b)], hi);
     } else if (bc == 1) {
-        atomic_add_f32(&f[slab_to_index(d, axis, g + l, a, b)], -lo);
-        atomic_add_f32(&f[slab_to_index(d, axis, n + l, a, b)], -hi);
+        atomic_add_t(&f[slab_to_index(d, axis, g + l, a, b)], -lo);
+        atomic_add_t(&f[slab_to_index(d, axis, n + l, a, b)], -hi);
     }
 }
 
 // ghost_cells.py:218-235: fold element t of a reduced (single-cell) axis.
-__device__ inline void fold_reduced_elem(const PicDesc& d, float* f, int axis, int bc, int64_t t) {
+template <class T>
+__device__ inline void fold_reduced_elem(const PicDesc& d, T* f, int axis, int bc, int64_t t) {
     int g = d.g;
     int da, db;
     other_dims(d, axis, da, db);
     int b = (int)(t % db);
     int a = (int)(t / db);
-    float lower = 0.0f, upper = 0.0f;
+    T lower = (T)0, upper = (T)0;
     for (int l = 0; l < g; ++l) {
         int64_t i0 = slab_to_index(d, axis, l, a, b);
         int64_t i1 = slab_to_index(d, axis, 1 + g + l, a, b);
         lower += f[i0];
         upper += f[i1];
-        f[i0] = 0.0f;
-        f[i1] = 0.0f;
+        f[i0] = (T)0;
+        f[i1] = (T)0;
     }
-    float s = lower + upper;
+    T s = lower + upper;
     int64_t c = slab_to_index(d, axis, g, a, b);
     if (bc == 0) f[c] += s;
     else if (bc == 1) f[c] += -s;
@@ -99,28 +106,32 @@ __device__ inline void fold_reduced_elem(const PicDesc& d, float* f, int axis, i
 
 // Value of a refreshed copy of f at (i,j,k) where exactly one coordinate may
 // sit one cell outside the interior along `axis`.
-__device__ inline float refreshed_at(const float* __restrict__ f, const PicDesc& d, const int mode[3], int i, int j,
-                                     int k, int axis) {
+template <class T>
+__device__ inline T refreshed_at(const T* __restrict__ f, const PicDesc& d, const int mode[3], int i, int j, int k,
+                                 int axis) {
     int c[3] = {i, j, k};
     int src = ghost_source(c[axis], d.n[axis], d.g, mode[axis]);
-    if (src < 0) return 0.0f;
+    if (src < 0) return (T)0;
     c[axis] = src;
     return f[fidx(d, c[0], c[1], c[2])];
 }
 
-struct YeeConst {
-    float hdt, dt, c2, inv_eps, id[3];
+template <class T>
+struct YeeConstT {
+    T hdt, dt, c2, inv_eps, id[3];
     int mode[3];
 };
+using YeeConst = YeeConstT<float>;
 
-__host__ __device__ inline YeeConst yee_const(const PicDesc& d) {
-    YeeConst y;
-    y.hdt = (float)(d.dt / 2);
-    y.dt = (float)d.dt;
-    y.c2 = (float)(d.C * d.C);
-    y.inv_eps = (float)(1.0 / d.eps);
+template <class T = float>
+__host__ __device__ inline YeeConstT<T> yee_const(const PicDesc& d) {
+    YeeConstT<T> y;
+    y.hdt = (T)(d.dt / 2);
+    y.dt = (T)d.dt;
+    y.c2 = (T)(d.C * d.C);
+    y.inv_eps = (T)(1.0 / d.eps);
     for (int a = 0; a < 3; ++a) {
-        y.id[a] = (float)(1.0 / d.d[a]);
+        y.id[a] = (T)(1.0 / d.d[a]);
         y.mode[a] = refresh_mode(d, a, 0);
     }
     return y;
@@ -128,19 +139,20 @@ __host__ __device__ inline YeeConst yee_const(const PicDesc& d) {
 
 // first_order_yee.py:96-142 at interior array point (i, j, k):
 // B_out = B_in - (dt/2) curl E with forward differences.
-__device__ inline void yee_b_point(const PicDesc& d, const YeeConst& y, const float* const E[3],
-                                   const float* const B[3], float* const out[3], int i, int j, int k) {
+template <class T>
+__device__ inline void yee_b_point(const PicDesc& d, const YeeConstT<T>& y, const T* const E[3], const T* const B[3],
+                                   T* const out[3], int i, int j, int k) {
     int64_t c = fidx(d, i, j, k);
-    float ex = E[0][c], ey = E[1][c], ez = E[2][c];
-    float ez_yp = refreshed_at(E[2], d, y.mode, i, j + 1, k, 1);
-    float ey_zp = refreshed_at(E[1], d, y.mode, i, j, k + 1, 2);
-    float ex_zp = refreshed_at(E[0], d, y.mode, i, j, k + 1, 2);
-    float ez_xp = refreshed_at(E[2], d, y.mode, i + 1, j, k, 0);
-    float ey_xp = refreshed_at(E[1], d, y.mode, i + 1, j, k, 0);
-    float ex_yp = refreshed_at(E[0], d, y.mode, i, j + 1, k, 1);
-    float curl_x = (ez_yp - ez) * y.id[1] - (ey_zp - ey) * y.id[2];
-    float curl_y = (ex_zp - ex) * y.id[2] - (ez_xp - ez) * y.id[0];
-    float curl_z = (ey_xp - ey) * y.id[0] - (ex_yp - ex) * y.id[1];
+    T ex = E[0][c], ey = E[1][c], ez = E[2][c];
+    T ez_yp = refreshed_at(E[2], d, y.mode, i, j + 1, k, 1);
+    T ey_zp = refreshed_at(E[1], d, y.mode, i, j, k + 1, 2);
+    T ex_zp = refreshed_at(E[0], d, y.mode, i, j, k + 1, 2);
+    T ez_xp = refreshed_at(E[2], d, y.mode, i + 1, j, k, 0);
+    T ey_xp = refreshed_at(E[1], d, y.mode, i + 1, j, k, 0);
+    T ex_yp = refreshed_at(E[0], d, y.mode, i, j + 1, k, 1);
+    T curl_x = (ez_yp - ez) * y.id[1] - (ey_zp - ey) * y.id[2];
+    T curl_y = (ex_zp - ex) * y.id[2] - (ez_xp - ez) * y.id[0];
+    T curl_z = (ey_xp - ey) * y.id[0] - (ex_yp - ex) * y.id[1];
     out[0][c] = B[0][c] - y.hdt * curl_x;
     out[1][c] = B[1][c] - y.hdt * curl_y;
     out[2][c] = B[2][c] - y.hdt * curl_z;
@@ -148,20 +160,20 @@ __device__ inline void yee_b_point(const PicDesc& d, const YeeConst& y, const fl
 
 // first_order_yee.py:12-72 at interior array point (i, j, k):
 // E_out = E_in + dt (C^2 curl B - J/eps) with backward differences.
-__device__ inline void yee_e_point(const PicDesc& d, const YeeConst& y, const float* const E[3],
-                                   const float* const B[3], const float* const J[3], float* const out[3], int i,
-                                   int j, int k) {
+template <class T>
+__device__ inline void yee_e_point(const PicDesc& d, const YeeConstT<T>& y, const T* const E[3], const T* const B[3],
+                                   const T* const J[3], T* const out[3], int i, int j, int k) {
     int64_t c = fidx(d, i, j, k);
-    float bx = B[0][c], by = B[1][c], bz = B[2][c];
-    float bz_ym = refreshed_at(B[2], d, y.mode, i, j - 1, k, 1);
-    float by_zm = refreshed_at(B[1], d, y.mode, i, j, k - 1, 2);
-    float bx_zm = refreshed_at(B[0], d, y.mode, i, j, k - 1, 2);
-    float bz_xm = refreshed_at(B[2], d, y.mode, i - 1, j, k, 0);
-    float by_xm = refreshed_at(B[1], d, y.mode, i - 1, j, k, 0);
-    float bx_ym = refreshed_at(B[0], d, y.mode, i, j - 1, k, 1);
-    float curl_x = (bz - bz_ym) * y.id[1] - (by - by_zm) * y.id[2];
-    float curl_y = (bx - bx_zm) * y.id[2] - (bz - bz_xm) * y.id[0];
-    float curl_z = (by - by_xm) * y.id[0] - (bx - bx_ym) * y.id[1];
+    T bx = B[0][c], by = B[1][c], bz = B[2][c];
+    T bz_ym = refreshed_at(B[2], d, y.mode, i, j - 1, k, 1);
+    T by_zm = refreshed_at(B[1], d, y.mode, i, j, k - 1, 2);
+    T bx_zm = refreshed_at(B[0], d, y.mode, i, j, k - 1, 2);
+    T bz_xm = refreshed_at(B[2], d, y.mode, i - 1, j, k, 0);
+    T by_xm = refreshed_at(B[1], d, y.mode, i - 1, j, k, 0);
+    T bx_ym = refreshed_at(B[0], d, y.mode, i, j - 1, k, 1);
+    T curl_x = (bz - bz_ym) * y.id[1] - (by - by_zm) * y.id[2];
+    T curl_y = (bx - bx_zm) * y.id[2] - (bz - bz_xm) * y.id[0];
+    T curl_z = (by - by_xm) * y.id[0] - (bx - bx_ym) * y.id[1];
     out[0][c] = E[0][c] + (y.c2 * curl_x - J[0][c] * y.inv_eps) * y.dt;
     out[1][c] = E[1][c] + (y.c2 * curl_y - J[1][c] * y.inv_eps) * y.dt;
     out[2][c] = E[2][c] + (y.c2 * curl_z - J[2][c] * y.inv_eps) * y.dt;

@@ -342,7 +342,7 @@ __device__ inline void node_axis(double pos, double org, double sp, double dd, i
                                  NodeAxis& s) {
     int a = anchor_of(pos, org, sp, sf);
     double dn = pos - node_coord(a, sp, org);
-    double df = (pos - org) - __dmul_rn(a + 0.5, dd);   // face offset (J_from_rhov.py:139-141)
+    double df = face_offset(pos, org, a, dd);   // J_from_rhov.py:139-141
     double wn[3], wf[3];
     shape3(dn / dd, sf, wn);
     shape3(df / dd, sf, wf);

@@ -10,7 +10,8 @@ update, or the Higuera-Cary update (higuera_cary.py:57-113) when
 from __future__ import annotations
 
 from .. import _native as N
-from .._tensors import (as_field3, check_fields, make_desc, make_species, new_soa, particle_planes, pic_particles)
+from .._tensors import (as_field3, check_fields, is_f64, make_desc, make_species, new_soa, particle_planes,
+                        pic_particles)
 from ..particles.particle_class import TiledParticles
 
 
@@ -33,8 +34,8 @@ def particle_push(particles, species_config, E_tiles, B_tiles, static_parameters
     B_external = None if B_external is None else [as_field3(c, "B_ext") for c in B_external]
     Ee = None if E_external is None else N.p3(E_external)
     Be = None if B_external is None else N.p3(B_external)
-    check_fields(desc, *E3, *B3, *(E_external or ()), *(B_external or ()))
-    store, view = new_soa(S, P, xp[0].device)
-    N.call("pic_push", desc, sp, pic_particles(xp, up, act), N.p3(E3), N.p3(B3), Ee, Be,
+    check_fields(desc, *E3, *B3, *(E_external or ()), *(B_external or ()), dtype=xp[0].dtype)
+    store, view = new_soa(S, P, xp[0].device, xp[0].dtype)
+    N.call(N.op("pic_push", is_f64(xp[0])), desc, sp, pic_particles(xp, up, act), N.p3(E3), N.p3(B3), Ee, Be,
            N.p3([store[0], store[1], store[2]]), N.stream_handle())
     return TiledParticles(x=particles.x, u=view, active=particles.active)

@@ -63,7 +63,10 @@ class ElectrodynamicRunner:
         ext_E = [as_field3(c, "external E") for c in external_fields[0]]
         ext_B = [as_field3(c, "external B") for c in external_fields[1]]
         self._ext = (ext_E, ext_B)
-        check_fields(self.desc, *self.E[0], *self.B, *self.J, *ext_E, *ext_B)
+        # the runner's kernels are the FP32 ones (float64 states step through evolve.time_loop_electrodynamic)
+        if xp[0].dtype != torch.float32:
+            raise ValueError("ElectrodynamicRunner runs float32 particles and fields")
+        check_fields(self.desc, *self.E[0], *self.B, *self.J, *ext_E, *ext_B, dtype=torch.float32)
         self.Ee = None if all(is_all_zero(c) for c in ext_E) else N.p3(ext_E)
         self.Be = None if all(is_all_zero(c) for c in ext_B) else N.p3(ext_B)
         self.pml_b = self.pml_e = None

@@ -55,7 +55,7 @@ def solve_poisson_with_conjugate_gradient(rho, phi, static_parameters, dynamic_p
     desc = make_desc(static_parameters, dynamic_parameters)
     rho3 = as_field3(rho, "rho")
     out = as_field3(phi, "phi").clone()
-    check_fields(desc, rho3, out)
+    check_fields(desc, rho3, out, dtype=torch.float32)
     poisson_cg_into(desc, rho3, out, tol, max_iter, N.stream_handle(), rtol=rtol)
     return like_input(out, phi)
 
@@ -94,7 +94,7 @@ def calculate_tiled_electrostatic_fields(static_parameters, dynamic_parameters, 
     rho_t = as_field3(rho_tiles, "rho")
     rho3 = rho_t.new_empty(rho_t.shape)
     phi3 = as_field3(phi_tiles, "phi").clone()
-    check_fields(desc, rho_t, phi3)
+    check_fields(desc, rho_t, phi3, dtype=torch.float32)
     E3 = [rho_t.new_empty(rho_t.shape) for _ in range(3)]
     electrostatic_fields_into(desc, sp, pic_particles(xp, up, act), rho3, phi3, E3, static_parameters.current_filter,
                               N.stream_handle(), tol=tol, max_iter=max_iter, rtol=rtol)
@@ -114,7 +114,7 @@ def calculate_electrostatic_fields(static_parameters, dynamic_parameters, partic
     desc = make_desc(static_parameters, dynamic_parameters)
     rho3 = as_field3(rho, "rho")
     phi3 = as_field3(phi, "phi").clone()
-    check_fields(desc, rho3, phi3)
+    check_fields(desc, rho3, phi3, dtype=torch.float32)
     s = N.stream_handle()
     poisson_cg_into(desc, rho3, phi3, tol, max_iter, s, rtol=rtol)
     phi_boundaries_into(desc, phi3, s)

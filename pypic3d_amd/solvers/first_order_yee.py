@@ -17,7 +17,7 @@ import ctypes
 import torch
 
 from .. import _native as N
-from .._tensors import as_field3, check_fields, like_input, make_desc
+from .._tensors import as_field3, check_fields, is_f64, like_input, make_desc
 
 
 def _check_pml(pml_state):
@@ -54,17 +54,21 @@ def pml_struct(mem6, sigma3):
 
 def yee_b(desc, E3, B3, out3, do_filter, stream, scratch=None, pml=None):
     """B half step on 3-D tensors into out3 (out3 must not alias E3).  pml:
-    None or (memory tensors[6], profile tensors[3]), memory updated in place."""
+    None or (memory tensors[6], profile tensors[3]), memory updated in place.
+    float64 fields run the FP64 path (without PML)."""
+    f64 = is_f64(E3[0])
+    if f64 and pml is not None:
+        raise ValueError("the PML runs in FP32 (float32 fields)")
     if pml is None:
-        N.call("pic_yee_update_b", desc, N.p3(E3), N.p3(B3), N.p3(out3), stream)
+        N.call(N.op("pic_yee_update_b", f64), desc, N.p3(E3), N.p3(B3), N.p3(out3), stream)
     else:
         N.call("pic_yee_update_b_pml", desc, N.p3(E3), N.p3(B3), N.p3(out3), ctypes.byref(pml_struct(*pml)),
                stream)
     if do_filter and desc.alpha != 1.0:
         tmp = scratch if scratch is not None else [c.clone() for c in out3]
         for c, t in zip(out3, tmp):
-            N.call("pic_filter_digital", desc, N.ptr(c), N.ptr(t), float(desc.alpha), stream)
-            N.call("pic_halo_refresh", desc, N.ptr(t), 0, stream)
+            N.call(N.op("pic_filter_digital", f64), desc, N.ptr(c), N.ptr(t), float(desc.alpha), stream)
+            N.call(N.op("pic_halo_refresh", f64), desc, N.ptr(t), 0, stream)
             c.copy_(t)
     return out3
 
@@ -72,23 +76,26 @@ def yee_b(desc, E3, B3, out3, do_filter, stream, scratch=None, pml=None):
 def yee_e(desc, E3, B3, J3, out3, stream, scratch=None, pml=None):
     """E full step on 3-D tensors into out3 (out3 must not alias B3 or J3).
     pml as for yee_b."""
+    f64 = is_f64(E3[0])
+    if f64 and pml is not None:
+        raise ValueError("the PML runs in FP32 (float32 fields)")
     if pml is None:
-        N.call("pic_yee_update_e", desc, N.p3(E3), N.p3(B3), N.p3(J3), N.p3(out3), stream)
+        N.call(N.op("pic_yee_update_e", f64), desc, N.p3(E3), N.p3(B3), N.p3(J3), N.p3(out3), stream)
     else:
         N.call("pic_yee_update_e_pml", desc, N.p3(E3), N.p3(B3), N.p3(J3), N.p3(out3),
                ctypes.byref(pml_struct(*pml)), stream)
     if desc.alpha != 1.0:
         tmp = scratch if scratch is not None else [c.clone() for c in out3]
         for c, t in zip(out3, tmp):
-            N.call("pic_filter_digital", desc, N.ptr(c), N.ptr(t), float(desc.alpha), stream)
+            N.call(N.op("pic_filter_digital", f64), desc, N.ptr(c), N.ptr(t), float(desc.alpha), stream)
             c.copy_(t)
     walls = {0: (1, 2), 1: (0, 2), 2: (0, 1)}
     for axis in range(3):
         if desc.bc[axis] == 1:
             for comp in walls[axis]:
-                N.call("pic_zero_boundary", desc, N.ptr(out3[comp]), axis, stream)
+                N.call(N.op("pic_zero_boundary", f64), desc, N.ptr(out3[comp]), axis, stream)
     for c in out3:
-        N.call("pic_halo_refresh", desc, N.ptr(c), 0, stream)
+        N.call(N.op("pic_halo_refresh", f64), desc, N.ptr(c), 0, stream)
     return out3
 
 
@@ -99,7 +106,7 @@ def _pml_args(desc, pml_state, which):
     clones = [m.clone() for m in mem_in]
     mem3 = pml_memory3(desc, clones)
     prof3 = [as_field3(pr, "PML profile") for pr in pml_state[2]]
-    check_fields(desc, *prof3)
+    check_fields(desc, *prof3, dtype=torch.float32)
     return mem3, prof3, tuple(clones)
 
 
@@ -110,6 +117,8 @@ def update_B(E_tiles, B_tiles, static_parameters, dynamic_parameters, pml_state=
     E3 = [as_field3(c, "E") for c in E_tiles]
     B3 = [as_field3(c, "B") for c in B_tiles]
     check_fields(desc, *E3, *B3)
+    if pml_state is not None and is_f64(E3[0]):
+        raise ValueError("the PML runs in FP32 (float32 fields)")
     out = [b.new_empty(b.shape) for b in B3]
     pml = None
     if pml_state is not None:
@@ -128,6 +137,8 @@ def update_E(E_tiles, B_tiles, J_tiles, static_parameters, dynamic_parameters, p
     B3 = [as_field3(c, "B") for c in B_tiles]
     J3 = [as_field3(c, "J") for c in J_tiles]
     check_fields(desc, *E3, *B3, *J3)
+    if pml_state is not None and is_f64(E3[0]):
+        raise ValueError("the PML runs in FP32 (float32 fields)")
     out = [e.new_empty(e.shape) for e in E3]
     pml = None
     if pml_state is not None:

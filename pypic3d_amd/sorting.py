@@ -27,6 +27,8 @@ def cell_keys(particles, species_config, static_parameters, dynamic_parameters):
     """int32 keys of shape (S, P)."""
     desc = make_desc(static_parameters, dynamic_parameters)
     xp, up, act = particle_planes(particles)
+    if xp[0].dtype != torch.float32:
+        raise ValueError("cell keys and the particle sort take float32 particles")
     S, P = xp[0].shape
     sp = make_species(species_config, P)
     keys = torch.empty((S, P), dtype=torch.int32, device=xp[0].device)

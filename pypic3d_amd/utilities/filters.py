@@ -7,14 +7,14 @@ stencil window ``g-1 .. -g+1``; ghost cells are left unchanged.
 from __future__ import annotations
 
 from .. import _native as N
-from .._tensors import as_field3, like_input, shape_desc
+from .._tensors import as_field3, is_f64, like_input, shape_desc
 
 
 def digital_filter(phi, alpha, num_guard_cells=1):
     """Centre alpha, six faces (1-alpha)/6 (filters.py:259-291)."""
     f = as_field3(phi, "phi")
     out = f.clone()
-    N.call("pic_filter_digital", shape_desc(f.shape, int(num_guard_cells)), N.ptr(f), N.ptr(out),
+    N.call(N.op("pic_filter_digital", is_f64(f)), shape_desc(f.shape, int(num_guard_cells)), N.ptr(f), N.ptr(out),
            float(alpha), N.stream_handle())
     return like_input(out, phi)
 
@@ -23,7 +23,7 @@ def bilinear_filter(phi, num_guard_cells=1):
     """(1,2,1)^3 / 64 (filters.py:234-256)."""
     f = as_field3(phi, "phi")
     out = f.clone()
-    N.call("pic_filter_bilinear", shape_desc(f.shape, int(num_guard_cells)), N.ptr(f), N.ptr(out),
+    N.call(N.op("pic_filter_bilinear", is_f64(f)), shape_desc(f.shape, int(num_guard_cells)), N.ptr(f), N.ptr(out),
            N.stream_handle())
     return like_input(out, phi)
 

@@ -27,16 +27,17 @@ def compute_energy(particles, E, B, static_parameters, dynamic_parameters, speci
     sp = make_species(species_config, xp[0].shape[1])
     E3 = [as_field3(c, "E") for c in E]
     B3 = [as_field3(c, "B") for c in B]
-    check_fields(desc, *E3, *B3)
-    nbytes = int(N.lib().pic_energy_workspace_bytes(desc, sp))
+    check_fields(desc, *E3, *B3, dtype=xp[0].dtype)
+    f64 = xp[0].dtype == torch.float64
+    nbytes = 3 * 1024 * 8 if f64 else int(N.lib().pic_energy_workspace_bytes(desc, sp))
     dev = E3[0].device
     ws = _ENERGY_WS.get((str(dev), nbytes))
     if ws is None:
         ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
         _ENERGY_WS[(str(dev), nbytes)] = ws
     out = torch.empty(3, dtype=torch.float64, device=dev)
-    N.call("pic_energy", desc, sp, pic_particles(xp, up, act), N.p3(E3), N.p3(B3), N.ptr(out), N.ptr(ws),
-           ws.numel(), N.stream_handle())
+    N.call(N.op("pic_energy", f64), desc, sp, pic_particles(xp, up, act), N.p3(E3), N.p3(B3), N.ptr(out),
+           N.ptr(ws), ws.numel(), N.stream_handle())
     return out[0], out[1], out[2]
 
 
@@ -90,6 +91,8 @@ def compute_total_momentum(particles, species_config=None):
     if species_config is None:
         raise ValueError("species_config is required for the momentum")
     xp, up, act = particle_planes(particles)
+    if xp[0].dtype != torch.float32:
+        raise ValueError("compute_total_momentum takes float32 particles")
     sp = make_species(species_config, xp[0].shape[1])
     dev = xp[0].device
     ws = _diag_workspace(dev)
@@ -106,7 +109,7 @@ def compute_total_charge(rho, static_parameters, dynamic_parameters):
     the same way (single_particle_pipeline_test.py:429-431)."""
     desc = make_desc(static_parameters, dynamic_parameters)
     rho3 = as_field3(rho, "rho")
-    check_fields(desc, rho3)
+    check_fields(desc, rho3, dtype=torch.float32)
     ws = _diag_workspace(rho3.device)
     out = torch.empty(1, dtype=torch.float64, device=rho3.device)
     N.call("pic_total_charge", desc, N.ptr(rho3), N.ptr(out), N.ptr(ws), ws.numel(), N.stream_handle())

@@ -54,11 +54,11 @@ def make_parameters(N, wind=None, d=None, dt=None, cfl=0.5, g=2, shape_factor=1,
     return static, dyn
 
 
-def empty_fields(static, dynamic, device="cuda"):
+def empty_fields(static, dynamic, device="cuda", dtype=torch.float32):
     """Zero (E, B, J, rho, phi, external_fields, None, overflow) in the tiled shape."""
     g = int(static.guard_cells)
     shape = (1, 1, 1, int(dynamic.Nx) + 2 * g, int(dynamic.Ny) + 2 * g, int(dynamic.Nz) + 2 * g)
-    z = lambda: torch.zeros(shape, dtype=torch.float32, device=device)   # noqa: E731
+    z = lambda: torch.zeros(shape, dtype=dtype, device=device)   # noqa: E731
     E = (z(), z(), z())
     B = (z(), z(), z())
     J = (z(), z(), z())
@@ -80,10 +80,11 @@ class SpeciesSpec:
     mode: int = 1
 
 
-def plasma(static, dynamic, species, seed=0, device="cuda"):
+def plasma(static, dynamic, species, seed=0, device="cuda", dtype=torch.float32):
     """Cell-ordered particles: `ppc` per cell per species, uniform in the cell.
     A density perturbation (1 + A cos(2 pi m x / W)) along x is applied by the
-    inverse-CDF map of each particle's x inside the domain."""
+    inverse-CDF map of each particle's x inside the domain.  dtype float32,
+    or float64 for the FP64 path (the same draws, not rounded to FP32)."""
     N = (int(dynamic.Nx), int(dynamic.Ny), int(dynamic.Nz))
     d = (float(dynamic.dx), float(dynamic.dy), float(dynamic.dz))
     wind = (float(dynamic.x_wind), float(dynamic.y_wind), float(dynamic.z_wind))
@@ -92,8 +93,8 @@ def plasma(static, dynamic, species, seed=0, device="cuda"):
     gen.manual_seed(int(seed))
     S = len(species)
     P = max(sp.ppc for sp in species) * ncell
-    xs = torch.zeros((3, S, P), dtype=torch.float32, device=device)
-    us = torch.zeros((3, S, P), dtype=torch.float32, device=device)
+    xs = torch.zeros((3, S, P), dtype=dtype, device=device)
+    us = torch.zeros((3, S, P), dtype=dtype, device=device)
     act = torch.zeros((S, P), dtype=torch.bool, device=device)
     cells = torch.arange(ncell, device=device, dtype=torch.int64)
     ci = (cells // (N[1] * N[2]), (cells // N[2]) % N[1], cells % N[2])
@@ -107,9 +108,9 @@ def plasma(static, dynamic, species, seed=0, device="cuda"):
                 pos = _perturbed(pos, wind[0], sp.perturb, sp.mode)
             if N[a] == 1:
                 pos = (frac - 0.5) * d[a] * 0.8
-            xs[a, s, :n] = pos.to(torch.float32)
+            xs[a, s, :n] = pos.to(dtype)
             us[a, s, :n] = (sp.drift[a] + sp.vth[a] * torch.randn(n, generator=gen, device=device,
-                                                                    dtype=torch.float64)).to(torch.float32)
+                                                                    dtype=torch.float64)).to(dtype)
         act[s, :n] = True
     cfg = SpeciesConfig(
         charge=np.array([sp.charge for sp in species], dtype=np.float64),
