@@ -17,7 +17,10 @@ recorded on the stream the kernel runs on (the phase is the fused kernel plus
 its rare-list kernel, about 0.15 ms).  traffic: calibrated FETCH_SIZE +
 WRITE_SIZE bytes per launch from the kernel's committed rocprofv3 summary in
 profiles/ for this workload (null when there is none).
-cpu_baseline: the FP64 NumPy oracle (oracle/, one core) on a bounded sample.
+cpu_baseline: the FP64 NumPy oracle (oracle/) on a bounded sample of the same
+workload, its particles split over one worker process per host core.
+two_stream_fp64: the 64-cell FP64 two-stream configuration, 500 steps, on
+the GPU's FP64 path and on the oracle (one core).
 """
 
 import argparse
@@ -46,10 +49,35 @@ def measured_traffic(kernel, n, ppc):
     return None, None
 
 
-def cpu_baseline(n=32, ppc=40):
-    """FP64 NumPy oracle, one electrodynamic Esirkepov step of an n^3 x ppc warm plasma."""
+_POOL_STATE = {}
+
+
+def _oracle_chunk(bounds):
+    """One worker's share of an oracle step: push, raw Esirkepov deposit,
+    position update and particle boundaries for slots [lo, hi) (the inputs
+    are inherited from the parent at fork); returns the raw J and a checksum
+    of the new positions."""
+    from oracle import pic_oracle as O
+    lo, hi = bounds
+    st, sp, p = _POOL_STATE["st"], _POOL_STATE["sp"], _POOL_STATE["p"]
+    x, u, act = st.x[:, lo:hi], st.u[:, lo:hi], st.active[:, lo:hi]
+    un = O.particle_push(x, u, act, sp, st.E, st.B, p)
+    J = O.esirkepov_raw(x, un, act, sp, p)
+    xn = O.update_positions(x, un, act, sp, p.dt)
+    xn, un, act, _ = O.refresh_particles(xn, un, act, p)
+    return J, float(xn.sum())
+
+
+def cpu_baseline(n=48, ppc=32):
+    """The FP64 NumPy oracle on the host's cores: one electrodynamic
+    Esirkepov step of an n^3 x ppc warm plasma (the benchmark workload at a
+    bounded size), the particles split over one worker process per core
+    (push, deposit, position update, boundaries), the J partial sums folded
+    and the Yee updates done in the parent."""
+    import multiprocessing as mp
     import numpy as np
     from oracle import pic_oracle as O
+    cores = min(16, len(os.sched_getaffinity(0)))
     p = O.Params(N=(n, n, n), wind=(float(n),) * 3, g=2, dt=0.5 / 3, current_deposition="esirkepov")
     rng = np.random.default_rng(20261202)
     P = n ** 3 * ppc
@@ -57,14 +85,65 @@ def cpu_baseline(n=32, ppc=40):
     u = (rng.standard_normal((1, P, 3)) * 0.05).astype(np.float32).astype(np.float64)
     sp = O.Species(np.array([-1.0]), np.array([1.0]), np.array([0.36 / 32]), np.ones((1, 3), bool),
                    np.ones((1, 3), bool))
-    z = [np.zeros(p.shape) for _ in range(3)]
-    st = O.State(x, u, np.ones((1, P), bool), z, [c.copy() for c in z])
+    E = [O.refresh((rng.standard_normal(p.shape) * 0.01), p) for _ in range(3)]
+    B = [O.refresh((rng.standard_normal(p.shape) * 0.01), p) for _ in range(3)]
+    _POOL_STATE.update(st=O.State(x, u, np.ones((1, P), bool), E, B), sp=sp, p=p)
+    edges = np.linspace(0, P, cores + 1).astype(int)
+    ctx = mp.get_context("fork")
+    with ctx.Pool(cores) as pool:
+        t0 = time.perf_counter()
+        parts = pool.map(_oracle_chunk, list(zip(edges[:-1], edges[1:])))
+        J = [sum(pj[0][c] for pj in parts) for c in range(3)]
+        J = [O.refresh(O.fold(c, p, p.particle_bc), p, p.particle_bc) for c in J]
+        B1 = O.update_B(E, B, p)
+        E1 = O.update_E(E, B1, J, p)
+        O.update_B(E1, B1, p, do_filter=True)
+        dt = time.perf_counter() - t0
+    _POOL_STATE.clear()
+    return {"value": P / dt, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "host_cpus": os.cpu_count(),
+            "sample": f"FP64 NumPy oracle, one electrodynamic Esirkepov step of a {n}^3 x {ppc} ppc warm plasma "
+                      f"({P} particles) split over {cores} worker processes, {dt:.1f} s"}
+
+
+def two_stream_fp64(steps=500, oracle=True):
+    """The 64-cell two-stream configuration in FP64 for `steps` steps: the
+    GPU's FP64 path (time_loop_electrodynamic on float64 tensors) and the FP64
+    NumPy oracle on one core, as particle-steps/s."""
+    import numpy as np
+    import torch
+    from oracle import pic_oracle as O
+    from pypic3d_amd import workloads
+    from pypic3d_amd.evolve import time_loop_electrodynamic
+    static, dyn, specs = workloads.config_two_stream_1d()
+    parts, cfg = workloads.plasma(static, dyn, specs, seed=20261202, dtype=torch.float64)
+    fields = workloads.empty_fields(static, dyn, dtype=torch.float64)
+    n_part = int(parts.active.numel())
+    x = parts.x[0, 0, 0].cpu().numpy()
+    u = parts.u[0, 0, 0].cpu().numpy()
+    a = parts.active[0, 0, 0].cpu().numpy()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    O.time_loop_electrodynamic(st, sp, p)
-    dt = time.perf_counter() - t0
-    return {"value": P / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 NumPy oracle, one electrodynamic Esirkepov step, {n}^3 x {ppc} ppc "
-                      f"({P} particles), {dt:.1f} s"}
+    for _ in range(steps):
+        parts, fields = time_loop_electrodynamic(parts, cfg, fields, static, dyn)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    out = {"steps": steps, "particles": n_part, "gpu_ms_per_step": 1e3 * gpu_s / steps,
+           "gpu_particle_steps_per_s": n_part * steps / gpu_s}
+    if oracle:
+        p = O.Params(N=(64, 1, 1), wind=(float(dyn.x_wind), float(dyn.y_wind), float(dyn.z_wind)), g=2,
+                     dt=float(dyn.dt))
+        sp = O.Species(np.asarray(cfg.charge), np.asarray(cfg.mass), np.asarray(cfg.weight),
+                       np.asarray(cfg.update_x), np.asarray(cfg.update_u))
+        z = [np.zeros(p.shape) for _ in range(3)]
+        st = O.State(x, u, a, z, [c.copy() for c in z])
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = O.time_loop_electrodynamic(st, sp, p)
+        cpu_s = time.perf_counter() - t0
+        out.update(oracle_cores=1, oracle_particle_steps_per_s=n_part * steps / cpu_s,
+                   oracle_s=cpu_s)
+    return out
 
 
 def _timed(fn, reps):
@@ -450,8 +529,11 @@ def main():
         print("[bench] adjoint step cost (differentiable two-stream)", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint()
         torch.cuda.empty_cache()
+    if not args.no_extras:
+        print("[bench] FP64 two-stream, 500 steps (GPU FP64 path and oracle)", file=sys.stderr, flush=True)
+        out["two_stream_fp64"] = two_stream_fp64(oracle=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
-        print("[bench] cpu baseline (NumPy oracle sample)", file=sys.stderr, flush=True)
+        print("[bench] cpu baseline (NumPy oracle sample on the host cores)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
 
