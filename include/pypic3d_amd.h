@@ -319,6 +319,11 @@ int pic_deposit_direct_f64(const PicDesc* desc, const PicSpecies* sp, const PicP
                            void* stream);
 int pic_deposit_rho_f64(const PicDesc* desc, const PicSpecies* sp, const PicParticlesF64* in, double* rho,
                         void* stream);
+/* wrap_periodic_position (boundary_conditions/grid_and_stencil.py:14-35) on n
+ * positions in place: mod(x + W/2, W) - W/2, a position at +W/2 kept there
+ * (evaluated in FP64; the FP32 variant rounds the result to FP32). */
+int pic_wrap_periodic(float* x, int64_t n, double wind, void* stream);
+int pic_wrap_periodic_f64(double* x, int64_t n, double wind, void* stream);
 /* workspace: at least 24576 bytes */
 int pic_energy_f64(const PicDesc* desc, const PicSpecies* sp, const PicParticlesF64* in, const double* const E[3],
                    const double* const B[3], double* out, void* workspace, int64_t workspace_bytes, void* stream);

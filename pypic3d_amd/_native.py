@@ -119,6 +119,8 @@ SIGNATURES = {
     "pic_deposit_direct_f64": (c_int, [DP, SP, PP, P3, c_vp]),
     "pic_deposit_rho_f64": (c_int, [DP, SP, PP, c_vp, c_vp]),
     "pic_energy_f64": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, c_i64, c_vp]),
+    "pic_wrap_periodic": (c_int, [c_vp, c_i64, c_dbl, c_vp]),
+    "pic_wrap_periodic_f64": (c_int, [c_vp, c_i64, c_dbl, c_vp]),
     "pic_batch_check": (c_int, [c_vp, c_i32, c_i64, c_i64]),
     "pic_batch_fused_push_esirkepov": (c_int, [c_vp, c_vp, c_i32, c_i32, c_i32, PP, P3, P3, PP, P3, c_vp]),
     "pic_batch_field_step": (c_int, [c_vp, c_i32, P3, P3, P3, P3, P3, c_vp]),

@@ -61,6 +61,8 @@ class AdjointRun:
         self.min_blocks = int(min_blocks)
         self.runner = None
         self.steps = 0
+        self.generation = 0      # counts forward runs; a loss's backward needs the run it came from
+        self.final = None
 
     # ------------------------------------------------------------ helpers
     def _fields(self, E0, B0):
@@ -112,6 +114,11 @@ class AdjointRun:
             if n % self.k == 0:
                 self.ckpt[n] = self._snapshot()
             self.runner.step()
+        self.generation += 1
+        # the final fields seed the adjoint; kept apart from the runner, whose
+        # buffers the reverse sweep's recomputation overwrites
+        r = self.runner
+        self.final = ([c.clone() for c in r.E[r.ecur]], [c.clone() for c in r.B])
         return self.loss()
 
     def loss(self):
@@ -128,8 +135,9 @@ class AdjointRun:
         dev = r.device
         s = N.stream_handle()
         dV = d.d[0] * d.d[1] * d.d[2]
-        lamE = torch.stack([self._interior(c).double() * (d.eps * dV) for c in r.E[r.ecur]]).contiguous()
-        lamB = torch.stack([self._interior(c).double() * (dV / d.mu) for c in r.B]).contiguous()
+        E_end, B_end = self.final
+        lamE = torch.stack([self._interior(c).double() * (d.eps * dV) for c in E_end]).contiguous()
+        lamB = torch.stack([self._interior(c).double() * (dV / d.mu) for c in B_end]).contiguous()
         lamJ = torch.empty_like(lamE)
         work = torch.empty((6,) + tuple(lamE.shape[1:]), dtype=torch.float64, device=dev)
         lx = torch.zeros((3, r.S, r.P), dtype=torch.float64, device=dev)
@@ -184,11 +192,15 @@ class FieldEnergyLoss(torch.autograd.Function):
     def forward(ctx, x0, v0, E0, B0, run: AdjointRun, steps: int):
         loss = run.forward(x0.detach(), v0.detach(), tuple(E0.detach()), tuple(B0.detach()), steps)
         ctx.run = run
+        ctx.generation = run.generation
         ctx.meta = (x0.dtype, v0.dtype, E0.dtype, B0.dtype, E0.shape)
         return loss.detach().clone()
 
     @staticmethod
     def backward(ctx, grad):
+        if ctx.run.generation != ctx.generation:
+            raise RuntimeError("this AdjointRun was run forward again after the loss was computed; its "
+                               "checkpoints now belong to the later run (use one AdjointRun per loss)")
         lx, lv, lE, lB = ctx.run.backward()
         xd, vd, ed, bd, gshape = ctx.meta
         g = grad.to(torch.float64)

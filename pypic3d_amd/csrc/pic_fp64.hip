@@ -608,3 +608,40 @@ int pic_energy_f64(const PicDesc* desc, const PicSpecies* sp, const PicParticles
 }
 
 }  // extern "C"
+
+namespace {
+
+// wrap_periodic_position (grid_and_stencil.py:14-35) on an array of
+// positions: mod(x + W/2, W) - W/2 for every element, +W/2 kept at +W/2.
+template <class T>
+__global__ void wrap_periodic_kernel(T* __restrict__ x, int64_t n, double wind) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = (double)x[i], u = 0.0;
+    bool act = true;
+    axis_boundary_d(0, wind, v, u, act);
+    x[i] = (T)v;
+}
+
+template <class T>
+int wrap_periodic_t(T* x, int64_t n, double wind, hipStream_t s) {
+    PIC_REQUIRE((x || n == 0) && n >= 0 && wind > 0.0);
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(wrap_periodic_kernel<T>, dim3(grid_1d(n, kBlock)), dim3(kBlock), 0, s, x, n, wind);
+    PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pic_wrap_periodic(float* x, int64_t n, double wind, void* stream) {
+    return wrap_periodic_t(x, n, wind, as_stream(stream));
+}
+
+int pic_wrap_periodic_f64(double* x, int64_t n, double wind, void* stream) {
+    return wrap_periodic_t(x, n, wind, as_stream(stream));
+}
+
+}  // extern "C"

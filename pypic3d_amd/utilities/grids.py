@@ -11,11 +11,11 @@ from __future__ import annotations
 
 import numpy as np
 
+from ..boundary_conditions.grid_and_stencil import build_collocated_axis, build_staggered_axis
+
 
 def _axis(minimum, d, n, half):
-    start = minimum - (0.5 * d if half else d)
-    stop = minimum + (n + 0.5) * d if half else minimum + n * d
-    return np.linspace(start, stop, int(n) + 2)
+    return build_staggered_axis(minimum, d, n) if half else build_collocated_axis(minimum, d, n)
 
 
 def build_collocated_grid(dynamic_parameters):

@@ -191,3 +191,25 @@ def test_reverse_mode_push_matches_the_tangent_passes(relativistic):
         grads.append([t.cpu().numpy() for t in run.backward()])
     for a, b in zip(*grads):
         assert np.max(np.abs(a - b)) <= 1e-5 * max(np.max(np.abs(b)), 1e-30)
+
+
+def test_reused_run_is_refused_and_backward_can_repeat():
+    """A second forward on the same AdjointRun invalidates the first loss's
+    backward (RuntimeError instead of gradients of the wrong trajectory); a
+    repeated backward of one loss (retain_graph) gives the same gradients."""
+    static, dyn, cfg = _setup(N_=(10, 8, 1))
+    d_ = torch.tensor(0.1, dtype=torch.float64, requires_grad=True)
+    x0, v0 = two_stream_state(static, dyn, 2, d_, 0.05, vth=0.01, seed=5, device="cpu")
+    x0c, v0c = x0.cuda(), v0.cuda()
+    g = int(static.guard_cells)
+    shape = (3, int(dyn.Nx) + 2 * g, int(dyn.Ny) + 2 * g, int(dyn.Nz) + 2 * g)
+    E0 = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    act = torch.ones((2, x0.shape[2]), dtype=torch.bool, device="cuda")
+    run = AdjointRun(static, dyn, cfg, act, checkpoint_every=3)
+    loss = field_energy_loss(x0c, v0c, E0, E0.clone(), run, 7)
+    g1 = torch.autograd.grad(loss, [d_], retain_graph=True)[0]
+    g2 = torch.autograd.grad(loss, [d_], retain_graph=True)[0]
+    assert float(g1) == float(g2)
+    field_energy_loss(x0c * 1.0001, v0c, E0, E0.clone(), run, 7)
+    with pytest.raises(RuntimeError):
+        torch.autograd.grad(loss, [d_])
