@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define PIC_ABI_VERSION 8
+#define PIC_ABI_VERSION 9
 #define PIC_MAX_SPECIES 16
 
 /* Scalar run description: StaticParameters + DynamicParameters fields the hot
@@ -229,6 +229,19 @@ int pic_tile_keys(const PicDesc* desc, const PicSpecies* sp, const PicParticles*
  * tiles: the inactive tail); starts[count*(tiles+1)] = count*slots. */
 int pic_tile_ranges(const PicDesc* desc, const PicSpecies* sp, int32_t tile, const int32_t* keys_sorted,
                     int32_t* starts, void* stream);
+/* Counting re-bin of the particles into tile-key order (the order
+ * pic_tile_keys + a sort produce, with the order inside one key unspecified):
+ * out receives the particles of in grouped by ascending tile key, starts the
+ * slot ranges of pic_tile_ranges, perm the source slot of every output slot.
+ * A counting pass (keys aggregated per 1024-slot chunk in an LDS hash table,
+ * one atomic per distinct key and chunk), a scan of the counts, a pass that
+ * writes perm from each chunk's reserved positions, and a gather of the
+ * particles through perm.  Replaces pic_tile_keys + pic_sort_permutation +
+ * pic_permute_particles + pic_tile_ranges.  workspace: the query below. */
+int64_t pic_tile_rebin_workspace_bytes(const PicDesc* desc, const PicSpecies* sp, int32_t tile);
+int pic_tile_rebin(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, int32_t tile,
+                   PicParticles* out, int32_t* starts, int32_t* perm, void* workspace, int64_t workspace_bytes,
+                   void* stream);
 int64_t pic_sort_workspace_bytes(int64_t n);
 /* Stable radix sort of (key, slot) pairs: perm = stable argsort(keys);
  * keys_sorted receives the sorted keys. */

@@ -13,6 +13,8 @@
 
 #include "pic_common.h"
 
+extern "C" int pic_fill_f32(float* ptr, int64_t n, float value, void* stream);
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -49,17 +51,11 @@ struct TileKeyConst {
     int nt[3], shift;
 };
 
-__global__ void tile_keys_kernel(PicDesc d, PicSpecies sp, PicParticles in, int T, TileKeyConst kc, uint32_t* keys,
-                                 int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t u = (uint32_t)i;     // n < 2^31 is checked at launch
-    int s = species_of(i, sp.slots, sp.count);
+__device__ inline uint32_t tile_key(const PicDesc& d, int T, const TileKeyConst& kc, int s, bool act,
+                                    const float x[3]) {
     const int64_t vol = (int64_t)T * T * T;
     const int64_t span = (int64_t)kc.nt[0] * kc.nt[1] * kc.nt[2] * vol;
     int64_t key = span;
-    const bool act = in.active[u] != 0;
-    const float x[3] = {in.x[0][u], in.x[1][u], in.x[2][u]};
     if (act) {
         int c[3], tq[3], tr[3];
 #pragma unroll
@@ -74,7 +70,180 @@ __global__ void tile_keys_kernel(PicDesc d, PicSpecies sp, PicParticles in, int 
         const int64_t local = ((int64_t)tr[0] * T + tr[1]) * T + tr[2];
         key = tile * vol + local;
     }
-    keys[u] = (uint32_t)(s * (span + 1) + key);
+    return (uint32_t)(s * (span + 1) + key);
+}
+
+__global__ void tile_keys_kernel(PicDesc d, PicSpecies sp, PicParticles in, int T, TileKeyConst kc, uint32_t* keys,
+                                 int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t u = (uint32_t)i;     // n < 2^31 is checked at launch
+    const float x[3] = {in.x[0][u], in.x[1][u], in.x[2][u]};
+    keys[u] = tile_key(d, T, kc, species_of(i, sp.slots, sp.count), in.active[u] != 0, x);
+}
+
+// ---- counting re-bin by tile key -------------------------------------------
+// A block takes kRebinChunk consecutive slots of the current order and
+// aggregates their keys in an LDS hash table (nearly sorted input has a few
+// dozen distinct keys per chunk, plus the particles that changed cell), so the
+// global atomics are one per distinct key per chunk.  Pass 1 counts the keys
+// and stores them; after the scan, pass 2 reserves each key's positions for
+// the chunk (one atomic per key) and writes the inverse permutation
+// perm[dst] = src; the particles are then gathered through perm (reads of
+// nearby slots, coalesced writes).  The order inside one key follows the
+// atomics and is not reproducible; the kernels do not depend on it.
+
+constexpr int kRebinItems = 4;
+constexpr int kRebinChunk = kBlock * kRebinItems;
+constexpr int kHashSize = 2 * kRebinChunk;          // load factor <= 1/2
+constexpr uint32_t kEmpty = 0xffffffffu;
+
+__device__ inline int hash_slot(uint32_t key) { return (int)((key * 2654435761u) >> 21) & (kHashSize - 1); }
+
+// slot of key in the table (inserting it), by linear probing
+__device__ inline int hash_insert(uint32_t* tkey, uint32_t key) {
+    int h = hash_slot(key);
+    for (;;) {
+        const uint32_t old = atomicCAS(&tkey[h], kEmpty, key);
+        if (old == kEmpty || old == key) return h;
+        h = (h + 1) & (kHashSize - 1);
+    }
+}
+
+__global__ void rebin_count_kernel(PicDesc d, PicSpecies sp, PicParticles in, int T, TileKeyConst kc,
+                                   uint32_t* __restrict__ count, uint32_t* __restrict__ keys, int64_t n) {
+    __shared__ uint32_t tkey[kHashSize];
+    __shared__ uint32_t tcnt[kHashSize];
+    for (int e = threadIdx.x; e < kHashSize; e += kBlock) {
+        tkey[e] = kEmpty;
+        tcnt[e] = 0u;
+    }
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * kRebinChunk;
+    for (int k = 0; k < kRebinItems; ++k) {
+        const int64_t i = b0 + (int64_t)k * kBlock + threadIdx.x;
+        if (i >= n) break;
+        const float x[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
+        const uint32_t key = tile_key(d, T, kc, species_of(i, sp.slots, sp.count), in.active[i] != 0, x);
+        keys[i] = key;
+        atomicAdd(&tcnt[hash_insert(tkey, key)], 1u);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kHashSize; e += kBlock)
+        if (tcnt[e]) atomicAdd(&count[tkey[e]], tcnt[e]);
+}
+
+__global__ void rebin_perm_kernel(const uint32_t* __restrict__ keys, uint32_t* __restrict__ pos,
+                                  int32_t* __restrict__ perm, int64_t n) {
+    __shared__ uint32_t tkey[kHashSize];
+    __shared__ uint32_t tcnt[kHashSize];
+    for (int e = threadIdx.x; e < kHashSize; e += kBlock) {
+        tkey[e] = kEmpty;
+        tcnt[e] = 0u;
+    }
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * kRebinChunk;
+    int slot[kRebinItems];
+    uint32_t rank[kRebinItems];
+    for (int k = 0; k < kRebinItems; ++k) {
+        const int64_t i = b0 + (int64_t)k * kBlock + threadIdx.x;
+        slot[k] = -1;
+        if (i < n) {
+            slot[k] = hash_insert(tkey, keys[i]);
+            rank[k] = atomicAdd(&tcnt[slot[k]], 1u);
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kHashSize; e += kBlock)
+        if (tcnt[e]) tcnt[e] = atomicAdd(&pos[tkey[e]], tcnt[e]);     // the chunk's base for the key
+    __syncthreads();
+    for (int k = 0; k < kRebinItems; ++k) {
+        if (slot[k] < 0) continue;
+        const int64_t i = b0 + (int64_t)k * kBlock + threadIdx.x;
+        perm[tcnt[slot[k]] + rank[k]] = (int32_t)i;
+    }
+}
+
+// Exclusive scan of n uint32 counts in place: per-block sums, a scan of the
+// sums, then each block scans its items from its offset.
+constexpr int kScanItems = 16;
+constexpr int kScanChunk = kBlock * kScanItems;
+
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        const uint32_t add = t >= off ? sh[t - off] : 0u;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    const uint32_t incl = sh[t];
+    total = sh[kBlock - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void scan_sums_kernel(const uint32_t* __restrict__ v, int64_t n, uint32_t* __restrict__ sums) {
+    __shared__ uint32_t sh[kBlock];
+    const int64_t b0 = (int64_t)blockIdx.x * kScanChunk;
+    uint32_t acc = 0u;
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t e = b0 + (int64_t)k * kBlock + threadIdx.x;
+        if (e < n) acc += v[e];
+    }
+    uint32_t total;
+    block_exclusive_scan(acc, sh, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// one block: exclusive scan of the block sums in place
+__global__ void scan_block_sums_kernel(uint32_t* __restrict__ sums, int64_t nb) {
+    __shared__ uint32_t sh[kBlock];
+    uint32_t carry = 0u;
+    for (int64_t base = 0; base < nb; base += kBlock) {
+        const int64_t e = base + threadIdx.x;
+        const uint32_t v = e < nb ? sums[e] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, sh, total);
+        if (e < nb) sums[e] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ void scan_apply_kernel(uint32_t* __restrict__ v, int64_t n, const uint32_t* __restrict__ sums) {
+    __shared__ uint32_t sh[kBlock];
+    const int64_t b0 = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
+    uint32_t loc[kScanItems];
+    uint32_t acc = 0u;
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t e = b0 + k;
+        loc[k] = e < n ? v[e] : 0u;
+        acc += loc[k];
+    }
+    uint32_t total;
+    uint32_t off = block_exclusive_scan(acc, sh, total) + sums[blockIdx.x];
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t e = b0 + k;
+        if (e < n) v[e] = off;
+        off += loc[k];
+    }
+}
+
+// starts[b] = first sorted slot of (species, tile) b = the exclusive prefix
+// at that pair's first key; starts[last] = n
+__global__ void rebin_starts_kernel(const uint32_t* __restrict__ pos, int64_t span, int64_t vol, int32_t tiles,
+                                    int32_t species, int64_t n, int32_t* __restrict__ starts) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nb = (int64_t)species * (tiles + 1);
+    if (b > nb) return;
+    if (b == nb) {
+        starts[b] = (int32_t)n;
+        return;
+    }
+    const int64_t s = b / (tiles + 1), t = b % (tiles + 1);
+    starts[b] = (int32_t)pos[s * (span + 1) + t * vol];
 }
 
 // starts[b] = first sorted position whose key >= the first key of tile b,
@@ -166,6 +335,79 @@ int pic_tile_ranges(const PicDesc* desc, const PicSpecies* sp, int32_t tile, con
                        reinterpret_cast<const uint32_t*>(keys_sorted), n, tiles * vol, vol, (int32_t)tiles,
                        (int32_t)sp->count, starts);
     PIC_CHECK_LAUNCH();
+    return 0;
+}
+
+static bool tile_key_const(const PicDesc* desc, int32_t tile, TileKeyConst& kc) {
+    if (!desc || tile < 1) return false;
+    tile_counts(*desc, tile, kc.nt);
+    kc.shift = -1;
+    for (int b = 0; b < 31; ++b)
+        if (tile == (1 << b)) kc.shift = b;
+    for (int a = 0; a < 3; ++a) {
+        kc.half_w[a] = 0.5 * desc->wind[a];
+        kc.inv_d[a] = 1.0 / desc->d[a];
+    }
+    return true;
+}
+
+static int64_t rebin_bins(const PicDesc* desc, const PicSpecies* sp, int32_t tile) {
+    const int64_t tiles = pic_tile_count(desc, tile);
+    if (tiles < 0 || !sp) return -1;
+    return (int64_t)sp->count * (tiles * tile * tile * tile + 1);
+}
+
+int64_t pic_tile_rebin_workspace_bytes(const PicDesc* desc, const PicSpecies* sp, int32_t tile) {
+    const int64_t bins = rebin_bins(desc, sp, tile);
+    if (bins < 0) return -1;
+    const int64_t blocks = (bins + kScanChunk - 1) / kScanChunk;
+    const int64_t n = (int64_t)sp->count * sp->slots;
+    return ((bins * 4 + 255) / 256) * 256 + ((blocks * 4 + 255) / 256) * 256 + n * 4 + 256;
+}
+
+int pic_tile_rebin(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, int32_t tile,
+                   PicParticles* out, int32_t* starts, int32_t* perm, void* workspace, int64_t workspace_bytes,
+                   void* stream) {
+    TileKeyConst kc;
+    PIC_REQUIRE(tile_key_const(desc, tile, kc) && particles_ok(sp, in) && out && starts);
+    const int64_t n = (int64_t)sp->count * sp->slots;
+    const int64_t bins = rebin_bins(desc, sp, tile);
+    PIC_REQUIRE(n < ((int64_t)1 << 31) && bins < ((int64_t)1 << 31));
+    PIC_REQUIRE(workspace && workspace_bytes >= pic_tile_rebin_workspace_bytes(desc, sp, tile));
+    const int64_t tiles = pic_tile_count(desc, tile);
+    const int64_t vol = (int64_t)tile * tile * tile;
+    hipStream_t s = as_stream(stream);
+    const int64_t nb = (int64_t)sp->count * (tiles + 1) + 1;
+    if (n > 0) {
+        PIC_REQUIRE(perm);
+        for (int a = 0; a < 3; ++a)
+            PIC_REQUIRE(out->x[a] && out->u[a] && out->x[a] != in->x[a] && out->u[a] != in->u[a]);
+        PIC_REQUIRE(out->active && out->active != in->active);
+    }
+    char* w = static_cast<char*>(workspace);
+    const int64_t blocks = (bins + kScanChunk - 1) / kScanChunk;
+    uint32_t* pos = reinterpret_cast<uint32_t*>(w);
+    uint32_t* sums = reinterpret_cast<uint32_t*>(w + ((bins * 4 + 255) / 256) * 256);
+    uint32_t* keys = reinterpret_cast<uint32_t*>(w + ((bins * 4 + 255) / 256) * 256 + ((blocks * 4 + 255) / 256) * 256);
+    int rc = pic_fill_f32(reinterpret_cast<float*>(pos), bins, 0.0f, stream);    // zero bits
+    if (rc) return rc;
+    const int64_t chunks = (n + kRebinChunk - 1) / kRebinChunk;
+    if (n > 0) {
+        hipLaunchKernelGGL(rebin_count_kernel, dim3((unsigned)chunks), dim3(kBlock), 0, s, *desc, *sp, *in, (int)tile,
+                           kc, pos, keys, n);
+        PIC_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(scan_sums_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, pos, bins, sums);
+    hipLaunchKernelGGL(scan_block_sums_kernel, dim3(1), dim3(kBlock), 0, s, sums, blocks);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, pos, bins, sums);
+    hipLaunchKernelGGL(rebin_starts_kernel, dim3(grid_1d(nb, kBlock)), dim3(kBlock), 0, s, pos, tiles * vol, vol,
+                       (int32_t)tiles, (int32_t)sp->count, n, starts);
+    PIC_CHECK_LAUNCH();
+    if (n > 0) {
+        hipLaunchKernelGGL(rebin_perm_kernel, dim3((unsigned)chunks), dim3(kBlock), 0, s, keys, pos, perm, n);
+        hipLaunchKernelGGL(permute_kernel, dim3(grid_1d(n, kBlock)), dim3(kBlock), 0, s, *in, *out, perm, n);
+        PIC_CHECK_LAUNCH();
+    }
     return 0;
 }
 

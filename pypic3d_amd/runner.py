@@ -191,12 +191,17 @@ class ElectrodynamicRunner:
         return best, times
 
     def _sort_buffers(self):
-        if getattr(self, "keys", None) is None:
-            n = self.S * self.P
-            dev = self.device
+        n = self.S * self.P
+        dev = self.device
+        if getattr(self, "perm", None) is None:
+            self.perm = torch.empty(n, dtype=torch.int32, device=dev)
+        if self.tile:
+            nbytes = int(N.lib().pic_tile_rebin_workspace_bytes(self.desc, self.sp, self.tile))
+            if getattr(self, "rebin_ws", None) is None or self.rebin_ws.numel() < nbytes:
+                self.rebin_ws = torch.empty(max(1, nbytes), dtype=torch.uint8, device=dev)
+        elif getattr(self, "keys", None) is None:
             self.keys = torch.empty(n, dtype=torch.int32, device=dev)
             self.keys_sorted = torch.empty(n, dtype=torch.int32, device=dev)
-            self.perm = torch.empty(n, dtype=torch.int32, device=dev)
             self.sort_ws = torch.empty(max(1, int(N.lib().pic_sort_workspace_bytes(n))), dtype=torch.uint8,
                                        device=dev)
 
@@ -224,25 +229,24 @@ class ElectrodynamicRunner:
         self.sort()
 
     def sort(self):
-        """Stable cell sort of the current particle buffers (into the other
-        set): reference cell order, or tile-major order in tile mode."""
+        """Re-order the current particle buffers (into the other set): in
+        tile mode a counting re-bin into tile-key order (the order inside one
+        cell is unspecified; results do not depend on it), otherwise the
+        stable sort into the reference's cell order.  perm[i] is the previous
+        slot of slot i."""
         self._sort_buffers()
         s = N.stream_handle()
         src, dst = self.cur, 1 - self.cur
         n = self.S * self.P
         if self.tile:
-            span = int(N.lib().pic_tile_count(self.desc, self.tile)) * self.tile ** 3
-            N.call("pic_tile_keys", self.desc, self.sp, self._pp(src), self.tile, N.ptr(self.keys), s)
-            bits = key_bits(self.S, span)
+            N.call("pic_tile_rebin", self.desc, self.sp, self._pp(src), self.tile, self._pp(dst), N.ptr(self.starts),
+                   N.ptr(self.perm), N.ptr(self.rebin_ws), self.rebin_ws.numel(), s)
         else:
             ncell = self.desc.n[0] * self.desc.n[1] * self.desc.n[2]
             N.call("pic_cell_keys", self.desc, self.sp, self._pp(src), N.ptr(self.keys), s)
-            bits = key_bits(self.S, ncell)
-        N.call("pic_sort_permutation", N.ptr(self.keys), N.ptr(self.keys_sorted), N.ptr(self.perm), n, bits,
-               N.ptr(self.sort_ws), self.sort_ws.numel(), s)
-        N.call("pic_permute_particles", self._pp(src), self._pp(dst), N.ptr(self.perm), n, s)
-        if self.tile:
-            N.call("pic_tile_ranges", self.desc, self.sp, self.tile, N.ptr(self.keys_sorted), N.ptr(self.starts), s)
+            N.call("pic_sort_permutation", N.ptr(self.keys), N.ptr(self.keys_sorted), N.ptr(self.perm), n,
+                   key_bits(self.S, ncell), N.ptr(self.sort_ws), self.sort_ws.numel(), s)
+            N.call("pic_permute_particles", self._pp(src), self._pp(dst), N.ptr(self.perm), n, s)
         self.cur = dst
         self.sorts_done += 1
 

@@ -196,7 +196,8 @@ def test_reverse_mode_push_matches_the_tangent_passes(relativistic):
 def test_reused_run_is_refused_and_backward_can_repeat():
     """A second forward on the same AdjointRun invalidates the first loss's
     backward (RuntimeError instead of gradients of the wrong trajectory); a
-    repeated backward of one loss (retain_graph) gives the same gradients."""
+    repeated backward of one loss (retain_graph) gives the same gradients up
+    to the atomic summation order of the recomputed forward steps."""
     static, dyn, cfg = _setup(N_=(10, 8, 1))
     d_ = torch.tensor(0.1, dtype=torch.float64, requires_grad=True)
     x0, v0 = two_stream_state(static, dyn, 2, d_, 0.05, vth=0.01, seed=5, device="cpu")
@@ -209,7 +210,8 @@ def test_reused_run_is_refused_and_backward_can_repeat():
     loss = field_energy_loss(x0c, v0c, E0, E0.clone(), run, 7)
     g1 = torch.autograd.grad(loss, [d_], retain_graph=True)[0]
     g2 = torch.autograd.grad(loss, [d_], retain_graph=True)[0]
-    assert float(g1) == float(g2)
+    # the recomputed segments differ only by the FP32 atomic summation order
+    assert abs(float(g1) - float(g2)) <= 1e-8 * abs(float(g1))
     field_energy_loss(x0c * 1.0001, v0c, E0, E0.clone(), run, 7)
     with pytest.raises(RuntimeError):
         torch.autograd.grad(loss, [d_])
