@@ -113,6 +113,75 @@ __device__ inline void atomic_add_f32(float* p, float v) {
     unsafeAtomicAdd(p, v);
 }
 
+// Lane i of the wave receives lane i - kOff's value (kOff < 16) inside its
+// aligned row of 16 lanes; lanes whose source leaves the row receive 0.
+template <int kOff>
+__device__ inline int dpp_row_shr(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x110 + kOff, 0xf, 0xf, true);
+}
+
+// Lane i receives lane i + 1's value inside its row of 16 (0 for the last lane).
+__device__ inline int dpp_row_shl1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x101, 0xf, 0xf, true); }
+
+// One step of a segmented inclusive scan inside rows of 16 lanes: `seg` is
+// set where a run head lies among the kOff lanes ending at this one; a lane
+// without one adds the partial sum of the lane kOff below it.
+// The DPP moves are evaluated unconditionally and selected afterwards: a
+// move placed under a lane condition would run with those lanes disabled, and
+// lanes reading from a disabled lane receive 0.
+template <int kOff, int kN>
+__device__ inline void row_scan_step(int r, int& seg, float (&v)[kN]) {
+    const bool take = seg == 0;
+    const int sb = dpp_row_shr<kOff>(seg);
+    const int below = r >= kOff ? sb : 1;
+#pragma unroll
+    for (int m = 0; m < kN; ++m) {
+        const float o = __int_as_float(dpp_row_shr<kOff>(__float_as_int(v[m])));
+        v[m] += take ? o : 0.0f;
+    }
+    seg |= below;
+}
+
+// Sums kN values over each run of equal keys in consecutive lanes inside
+// each aligned row of 16 lanes: afterwards the last lane of a run within its
+// row holds the run's row total, and the call returns true on exactly those
+// lanes (other lanes hold partial sums, which the caller discards).  Equal
+// keys in lanes that are not consecutive form separate runs.  Every lane of
+// the wave must execute it.
+template <int kN>
+__device__ inline bool row_run_sums(int key, float (&v)[kN]) {
+    const int r = (int)(threadIdx.x & 15u);
+    const int kprev = dpp_row_shr<1>(key), knext = dpp_row_shl1(key);
+    int seg = ((r == 0) | (kprev != key)) ? 1 : 0;    // run head
+    row_scan_step<1>(r, seg, v);
+    row_scan_step<2>(r, seg, v);
+    row_scan_step<4>(r, seg, v);
+    row_scan_step<8>(r, seg, v);
+    return (r == 15) | (knext != key);
+}
+
+// kP consecutive floats from 16-byte (kP = 4) or 8-byte (kP = 2) aligned memory
+template <int kP>
+__device__ inline void store_vec(float* p, const float v[kP]) {
+    if constexpr (kP == 4) *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    else *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+}
+
+template <int kP>
+__device__ inline void load_vec(const float* p, float v[kP]) {
+    if constexpr (kP == 4) {
+        const float4 q = *reinterpret_cast<const float4*>(p);
+        v[0] = q.x;
+        v[1] = q.y;
+        v[2] = q.z;
+        v[3] = q.w;
+    } else {
+        const float2 q = *reinterpret_cast<const float2*>(p);
+        v[0] = q.x;
+        v[1] = q.y;
+    }
+}
+
 // positive modulo (jnp.mod of an int); the in-range case skips the division
 __device__ inline int pmod(int a, int L) {
     if ((unsigned)a < (unsigned)L) return a;

@@ -231,26 +231,6 @@ __device__ inline void tile_particle_general(const PicDesc& d, const SpeciesK& k
 // rare[1]) without being stored; fused_tile_rare_kernel then runs it whole.
 // Keeping those paths out of this kernel keeps its register count low.
 
-template <int kP>
-__device__ inline void store_vec(float* p, const float v[kP]) {
-    if constexpr (kP == 4) *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
-    else *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
-}
-
-template <int kP>
-__device__ inline void load_vec(const float* p, float v[kP]) {
-    if constexpr (kP == 4) {
-        const float4 q = *reinterpret_cast<const float4*>(p);
-        v[0] = q.x;
-        v[1] = q.y;
-        v[2] = q.z;
-        v[3] = q.w;
-    } else {
-        const float2 q = *reinterpret_cast<const float2*>(p);
-        v[0] = q.x;
-        v[1] = q.y;
-    }
-}
 
 __device__ inline void rare_push(int32_t* rare, int64_t i) {
     const int p = atomicAdd(rare, 1);
@@ -607,7 +587,16 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
                 out.active[i] = (uint8_t)((oact >> (8 * j)) & 0xffu);
             }
         }
-        if (have) flush_cell<kB>(d, sink, h0, h1, h2, acc);
+        {
+            // the group's last cell: equal cells of neighbouring lanes are summed
+            // first, so one lane per cell and 16-lane row adds into the box (a
+            // cell outside the box keeps a key of its own)
+            const int bi = h0 - o[0], bj = h1 - o[1], bk = h2 - o[2];
+            const bool inbox = (unsigned)bi < (unsigned)(kB - 1) && (unsigned)bj < (unsigned)(kB - 1) &&
+                               (unsigned)bk < (unsigned)(kB - 1);
+            const int key = !have ? -1 : (inbox ? (bi * kB + bj) * kB + bk : -2 - (int)(threadIdx.x & 63u));
+            if (row_run_sums(key, acc) && have) flush_cell<kB>(d, sink, h0, h1, h2, acc);
+        }
         __syncthreads();
         // ---- queued face crossings (3-slot frame, Esirkepov.py:105-331)
         const int nq = min(sh.q_n, kCrossCap);
@@ -716,19 +705,66 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
         for (int m = 0; m < 24; ++m) acc[m] = 0.0f;
         int h0 = 0, h1 = 0, h2 = 0;
         bool have = false;
+        // kVec: OX/OU first hold the group's inputs (one kP-wide load per plane
+        // when the group lies inside the range) and take its outputs slot by
+        // slot, as in fused_tile_exact_kernel
         float OX[3][kP], OU[3][kP];
-        uint32_t oact = 0u;
+        uint32_t pact = 0u, oact = 0u;
+        const bool full = vec_ok && s0 >= begin && s0 + kP <= end;
+        if constexpr (kVec) {
+            if (full) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    load_vec<kP>(in.x[a] + s0, OX[a]);
+                    load_vec<kP>(in.u[a] + s0, OU[a]);
+                }
+                pact = *reinterpret_cast<const uint32_t*>(in.active + s0);
+            } else {
+#pragma unroll
+                for (int j = 0; j < kP; ++j) {
+                    const int64_t i = s0 + j;
+                    const bool inr = i >= begin && i < end;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        OX[a][j] = inr ? in.x[a][i] : 0.0f;
+                        OU[a][j] = inr ? in.u[a][i] : 0.0f;
+                    }
+                    pact |= (inr && in.active[i] != 0 ? 1u : 0u) << (8 * j);
+                }
+            }
+        }
 #pragma unroll 1
         for (int j = 0; j < kP; ++j) {
             const int64_t i = s0 + j;
             float ox[3] = {0.0f, 0.0f, 0.0f}, ou[3] = {0.0f, 0.0f, 0.0f};
             uint32_t oa = 0u;
+            float xf[3], v[3];
+            bool act0;
+            if constexpr (kVec) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    xf[a] = OX[a][0];
+                    v[a] = OU[a][0];
+#pragma unroll
+                    for (int q = 0; q + 1 < kP; ++q) {
+                        OX[a][q] = OX[a][q + 1];
+                        OU[a][q] = OU[a][q + 1];
+                    }
+                }
+                act0 = (pact & 0xffu) != 0u;
+                pact >>= 8;
+            }
             do {
-            if (i < begin || i >= end) break;
-            const bool act0 = in.active[i] != 0;
-            const float xf[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
-            float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
-            loads_together(xf, v);
+            if constexpr (!kVec) {
+                if (i < begin || i >= end) break;
+                act0 = in.active[i] != 0;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    xf[a] = in.x[a][i];
+                    v[a] = in.u[a][i];
+                }
+                loads_together(xf, v);
+            }
             if (!act0) break;
             int anc[3], anv[3];
             float dc[3], dv[3], nodec[3];
@@ -844,11 +880,6 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
             if constexpr (kVec) {
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-#pragma unroll
-                    for (int q = 0; q + 1 < kP; ++q) {
-                        OX[a][q] = OX[a][q + 1];
-                        OU[a][q] = OU[a][q + 1];
-                    }
                     OX[a][kP - 1] = ox[a];
                     OU[a][kP - 1] = ou[a];
                 }
@@ -862,9 +893,12 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
                 out.active[i] = (uint8_t)oa;
             }
         }
-        if (have) flush(h0, h1, h2, acc);
+        {
+            // the group's last cell, summed over equal cells of neighbouring lanes first
+            const int key = have ? ((h0 - o[0]) * kB + (h1 - o[1])) * kB + (h2 - o[2]) : -1;
+            if (row_run_sums(key, acc) && have) flush(h0, h1, h2, acc);
+        }
         if constexpr (!kVec) continue;
-        const bool full = vec_ok && s0 >= begin && s0 + kP <= end;
         if (full) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -1035,8 +1069,10 @@ int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const
     hipStream_t st = as_stream(stream);
     FieldPtrs fp = make_fields(E, B, E_ext, B_ext);
     int vec_ok = 1;
-    for (int a = 0; a < 3; ++a) vec_ok &= ((((uintptr_t)out->x[a]) | ((uintptr_t)out->u[a])) & 15u) == 0;
-    vec_ok &= (((uintptr_t)out->active) & 3u) == 0;
+    for (int a = 0; a < 3; ++a)
+        vec_ok &= ((((uintptr_t)out->x[a]) | ((uintptr_t)out->u[a]) | ((uintptr_t)in->x[a]) |
+                    ((uintptr_t)in->u[a])) & 15u) == 0;
+    vec_ok &= ((((uintptr_t)out->active) | ((uintptr_t)in->active)) & 3u) == 0;
     hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, workspace);
 #define PIC_DIRECT_LAUNCH(T, MB, V)                                                                               \
     hipLaunchKernelGGL((fused_tile_direct_kernel<T, MB, V>), dim3((unsigned)blocks), dim3(kThreads), 0, st, *desc,   \

@@ -243,16 +243,24 @@ __global__ __launch_bounds__(kBlock) void tiled_deposit_kernel(PicDesc d, PicSpe
 // registers while the anchor stays and added into the LDS box when it
 // changes.  Anchors come from exact_anchors (FP32, equal to the FP64
 // evaluation); particles outside the exact range or the box take the general
-// per-particle deposit.
+// per-particle deposit after the group.
+//
+// A thread's four slots are read with one 16-byte load per plane (4 bytes of
+// activity flags) when the group lies inside the block's range and the planes
+// are aligned (vec_ok), so a wave reads 1 KiB of contiguous memory per plane;
+// the next chunk's group is loaded before the current one is deposited, so
+// the memory latency overlaps the arithmetic.
 template <int kC, int kT, bool kRho>
 __global__ __launch_bounds__(kBlock) void tiled_deposit_group_kernel(PicDesc d, PicSpecies sp, PicParticles in,
                                                                      float* J0, float* J1, float* J2,
                                                                      const int32_t* __restrict__ starts, int tiles,
-                                                                     int nt1, int nt2, ExactGrid eg) {
+                                                                     int nt1, int nt2, ExactGrid eg, int vec_ok) {
     constexpr int kB = kT + 5;
     constexpr int kV = kB * kB * kB;
     constexpr int kP = 4;
     constexpr int kN = 8 * kC;
+    constexpr int kU = kRho ? 1 : 3;       // velocity planes held per group (none read for rho)
+    constexpr int kChunk = kBlock * kP;
     __shared__ float box[kC * kV];
     const int b = blockIdx.x;
     const int64_t begin = starts[b], end = starts[b + 1];
@@ -277,33 +285,66 @@ __global__ __launch_bounds__(kBlock) void tiled_deposit_group_kernel(PicDesc d, 
             atomicAdd(p + c * kV + ((n >> 2) * kB + ((n >> 1) & 1)) * kB + (n & 1), acc[m]);
         }
     };
-    for (int64_t base = begin & ~(int64_t)(kP - 1); base < end; base += kBlock * kP) {
+    struct Group {
+        float x[3][kP], u[kU][kP];
+        uint32_t act;      // byte j: slot j is inside [begin, end) and active
+    };
+    auto load = [&](int64_t s0, Group& g) {
+        if (vec_ok && s0 >= begin && s0 + kP <= end) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) load_vec<kP>(in.x[a] + s0, g.x[a]);
+            if constexpr (!kRho) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) load_vec<kP>(in.u[a] + s0, g.u[a]);
+            }
+            g.act = *reinterpret_cast<const uint32_t*>(in.active + s0);
+        } else {
+            g.act = 0u;
+#pragma unroll
+            for (int j = 0; j < kP; ++j) {
+                const int64_t i = s0 + j;
+                const bool inr = i >= begin && i < end;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) g.x[a][j] = inr ? in.x[a][i] : 0.0f;
+                if constexpr (!kRho) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) g.u[a][j] = inr ? in.u[a][i] : 0.0f;
+                }
+                g.act |= (inr && in.active[i] != 0 ? 1u : 0u) << (8 * j);
+            }
+        }
+    };
+    int64_t base = begin & ~(int64_t)(kP - 1);
+    Group cur;
+    load(base + (int64_t)kP * threadIdx.x, cur);
+    for (; base < end; base += kChunk) {
         const int64_t s0 = base + (int64_t)kP * threadIdx.x;
+        Group nxt;
+        nxt.act = 0u;
+        if (base + kChunk < end) load(s0 + kChunk, nxt);
         float acc[kN];
 #pragma unroll
         for (int m = 0; m < kN; ++m) acc[m] = 0.0f;
         int h0 = 0, h1 = 0, h2 = 0;
         bool have = false;
-#pragma unroll 1
+        uint32_t general = 0u;
+#pragma unroll
         for (int j = 0; j < kP; ++j) {
-            const int64_t i = s0 + j;
-            if (i < begin || i >= end || !in.active[i]) continue;
-            float x[3];
+            if (((cur.act >> (8 * j)) & 0xffu) == 0u) continue;
             int an[3];
             float dn[3];
             bool fast = true;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                x[a] = in.x[a][i];
+                const float x = cur.x[a][j];
                 int av;
                 float nd, dv;
-                fast = fast && x[a] >= eg.xlo[a] && x[a] < eg.xhi[a];
-                exact_anchors(x[a], eg.org_c[a], eg.sp_c[a], eg.isp_c[a], eg.half_sp[a], an[a], dn[a], nd, av, dv);
+                fast = fast && x >= eg.xlo[a] && x < eg.xhi[a];
+                exact_anchors(x, eg.org_c[a], eg.sp_c[a], eg.isp_c[a], eg.half_sp[a], an[a], dn[a], nd, av, dv);
                 fast = fast && (unsigned)(an[a] - o[a]) <= (unsigned)(kB - 2);
             }
             if (!fast) {
-                if constexpr (kRho) rho_particle(d, q, in, i, sink);
-                else direct_particle(d, q, in, i, sink);
+                general |= 1u << j;
                 continue;
             }
             float wn[3][2], wf[3][2];
@@ -321,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void tiled_deposit_group_kernel(PicDesc d, 
 #pragma unroll
                 for (int n = 0; n < 8; ++n) val[n] = q * wn[0][n >> 2] * wn[1][(n >> 1) & 1] * wn[2][n & 1];
             } else {
-                const float qv[3] = {q * in.u[0][i], q * in.u[1][i], q * in.u[2][i]};
+                const float qv[3] = {q * cur.u[0][j], q * cur.u[1][j], q * cur.u[2][j]};
 #pragma unroll
                 for (int n = 0; n < 8; ++n) {
                     const int xi = n >> 2, yi = (n >> 1) & 1, zi = n & 1;
@@ -333,13 +374,23 @@ __global__ __launch_bounds__(kBlock) void tiled_deposit_group_kernel(PicDesc d, 
             const bool same = have && an[0] == h0 && an[1] == h1 && an[2] == h2;
             if (have && !same) flush(h0, h1, h2, acc);
 #pragma unroll
-            for (int m = 0; m < kN; ++m) acc[m] = same ? acc[m] + val[m] : val[m];
+            for (int m = 0; m < kN; ++m) acc[m] = (same ? acc[m] : 0.0f) + val[m];
             h0 = an[0];
             h1 = an[1];
             h2 = an[2];
             have = true;
         }
-        if (have) flush(h0, h1, h2, acc);
+        // the group's last cell: equal cells of neighbouring lanes are summed
+        // first, so one lane per cell and 16-lane row adds into the box
+        const int key = have ? ((h0 - o[0]) * kB + (h1 - o[1])) * kB + (h2 - o[2]) : -1;
+        if (row_run_sums(key, acc) && have) flush(h0, h1, h2, acc);
+        while (general) {
+            const int j = __builtin_ctz(general);
+            general &= general - 1u;
+            if constexpr (kRho) rho_particle(d, q, in, s0 + j, sink);
+            else direct_particle(d, q, in, s0 + j, sink);
+        }
+        cur = nxt;
     }
     __syncthreads();
     float* const J[3] = {J0, J1, J2};
@@ -438,7 +489,13 @@ static int tiled_deposit(const PicDesc* desc, const PicSpecies* sp, const PicPar
     for (int a = 0; a < 3 && grouped; ++a) grouped = desc->d[a] == desc->sp_c[a];
 #define PIC_TDG(C, T, R)                                                                                            \
     hipLaunchKernelGGL((tiled_deposit_group_kernel<C, T, R>), dim3((unsigned)blocks), dim3(kBlock), 0, st, *desc, \
-                       *sp, *in, J[0], j1, j2, starts, (int)tiles, nt[1], nt[2], eg)
+                       *sp, *in, J[0], j1, j2, starts, (int)tiles, nt[1], nt[2], eg, vec_ok)
+    int vec_ok = 1;
+    for (int a = 0; a < 3; ++a) {
+        vec_ok &= (((uintptr_t)in->x[a]) & 15u) == 0;
+        if (!rho) vec_ok &= (((uintptr_t)in->u[a]) & 15u) == 0;
+    }
+    vec_ok &= (((uintptr_t)in->active) & 3u) == 0;
     if (grouped) {
         if (rho) {
             if (tile == 8) PIC_TDG(1, 8, true);
