@@ -113,6 +113,17 @@ __device__ inline void atomic_add_f32(float* p, float v) {
     unsafeAtomicAdd(p, v);
 }
 
+// Logical block of a launch of `blocks` workgroups such that each of the 8
+// XCDs (which receive workgroups round-robin by id) runs one contiguous range
+// of logical blocks, in order: neighbouring tiles then share their halo nodes
+// through the same L2.  A bijection on [0, blocks).
+__device__ inline int xcd_block(int id, int blocks) {
+    constexpr int kXcds = 8;
+    const int q = blocks / kXcds, rem = blocks % kXcds;
+    const int x = id % kXcds, local = id / kXcds;
+    return x * q + min(x, rem) + local;
+}
+
 // Lane i of the wave receives lane i - kOff's value (kOff < 16) inside its
 // aligned row of 16 lanes; lanes whose source leaves the row receive 0.
 template <int kOff>

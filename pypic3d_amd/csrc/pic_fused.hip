@@ -222,7 +222,9 @@ __device__ inline void tile_particle_general(const PicDesc& d, const SpeciesK& k
 // owns kP consecutive slots; sorted slots of one cell are contiguous, so a
 // thread's particles mostly share a cell and their twelve regular-case values
 // are summed in registers and added into the LDS current box when the cell
-// changes and at the end of the group (no cross-lane reduction).  Fields are
+// changes; at the end of the group the lanes of each 16-lane row first sum
+// their values over equal cells (row_run_sums), so one lane per cell and row
+// adds into the box.  Fields are
 // gathered from the LDS field box.  Particles that cross a face go to the
 // block's crossing queue (3-slot deposit into the box after the chunk).
 // Anything else -- a particle outside the tile's field box or the exact
@@ -328,12 +330,17 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
     __shared__ CrossQueue sh;
     __shared__ float box[3 * kB3];
     __shared__ float fbox[6 * kFV];
-    const int b = blockIdx.x;
+    const int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int64_t begin = starts[b], end = starts[b + 1];
     if (begin >= end) return;
     const int t = b % (tg.tiles + 1);
     const int s = b / (tg.tiles + 1);          // blocks are grouped per species
     int o[3] = {0, 0, 0}, fo[3] = {0, 0, 0};
+    // interior: every particle the fast path handles (anchors inside the field
+    // box, cells c0 - 2 .. c0 + T + 1, a step of under one cell) stays inside
+    // [-W/2, W/2) on every axis, so its particle boundaries do nothing and are
+    // skipped (a block-uniform branch)
+    bool interior = t < tg.tiles;
     if (t < tg.tiles) {
         const int tx = t / (tg.nt[1] * tg.nt[2]), ty = (t / tg.nt[2]) % tg.nt[1], tz = t % tg.nt[2];
         const int c0[3] = {tx * kT, ty * kT, tz * kT};
@@ -341,20 +348,11 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
         for (int a = 0; a < 3; ++a) {
             o[a] = c0[a] + d.g - 1;
             fo[a] = c0[a] + d.g - 2;
+            interior = interior && c0[a] >= 3 && c0[a] + kT + 3 <= d.n[a];
         }
     }
     for (int e = threadIdx.x; e < 3 * kB3; e += kThreads) box[e] = 0.0f;
-    for (int e = threadIdx.x; e < 6 * kFV; e += kThreads) {
-        const int c = e / kFV, r = e % kFV;
-        const int i = pmod(fo[0] + r / (kFB * kFB), d.dims[0]), j = pmod(fo[1] + (r / kFB) % kFB, d.dims[1]);
-        const int k = pmod(fo[2] + r % kFB, d.dims[2]);
-        const int64_t id = fidx(d, i, j, k);
-        const float* F = c < 3 ? fp.E[c] : fp.B[c - 3];
-        const float* Fe = c < 3 ? fp.Ee[c] : fp.Be[c - 3];
-        float v = F[id];
-        if (Fe) v += Fe[id];
-        fbox[e] = v;
-    }
+    fill_field_box<kFB>(d, fp, fo, fbox);
     const BoxSink sink{{Jx, Jy, Jz}, box, {o[0], o[1], o[2]}, kB};
     const FieldBox fb{fbox, {fo[0], fo[1], fo[2]}, kFB};
     const SpeciesK k = species_constants(d, sp, s);
@@ -547,10 +545,12 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
             }
             // ---- position update + particle boundaries (particle_tile_communication.py)
             bool act = true;
+            if (!interior) {
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                if (d.pbc[a] == 0) periodic_wrap(d.wind[a], xn[a]);
-                else apply_axis_boundary(d.pbc[a], d.wind[a], xn[a], v[a], act);
+                for (int a = 0; a < 3; ++a) {
+                    if (d.pbc[a] == 0) periodic_wrap(d.wind[a], xn[a]);
+                    else apply_axis_boundary(d.pbc[a], d.wind[a], xn[a], v[a], act);
+                }
             }
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -611,13 +611,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
         }
         __syncthreads();
     }
-    float* const J[3] = {Jx, Jy, Jz};
-    for (int e = threadIdx.x; e < 3 * kB3; e += kThreads) {
-        const float v = box[e];
-        if (v == 0.0f) continue;
-        const int c = e / kB3, r = e % kB3;
-        deposit_at(J[c], d, o[0] + r / (kB * kB), o[1] + (r / kB) % kB, o[2] + r % kB, v);
-    }
+    flush_current_box<kB>(d, o, box, Jx, Jy, Jz);
 }
 
 // The rare list of fused_tile_exact_kernel: each listed particle runs the
@@ -661,7 +655,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
     constexpr int kChunk = kThreads * kP;
     __shared__ float box[3 * kB3];
     __shared__ float fbox[6 * kFV];
-    const int b = blockIdx.x;
+    const int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int64_t begin = starts[b], end = starts[b + 1];
     if (begin >= end) return;
     const int t = b % (tg.tiles + 1);
@@ -674,17 +668,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
         for (int a = 0; a < 3; ++a) o[a] = fo[a] = c0[a] + d.g - 2;
     }
     for (int e = threadIdx.x; e < 3 * kB3; e += kThreads) box[e] = 0.0f;
-    for (int e = threadIdx.x; e < 6 * kFV; e += kThreads) {
-        const int c = e / kFV, r = e % kFV;
-        const int i = pmod(fo[0] + r / (kFB * kFB), d.dims[0]), j = pmod(fo[1] + (r / kFB) % kFB, d.dims[1]);
-        const int k = pmod(fo[2] + r % kFB, d.dims[2]);
-        const int64_t id = fidx(d, i, j, k);
-        const float* F = c < 3 ? fp.E[c] : fp.B[c - 3];
-        const float* Fe = c < 3 ? fp.Ee[c] : fp.Be[c - 3];
-        float v = F[id];
-        if (Fe) v += Fe[id];
-        fbox[e] = v;
-    }
+    fill_field_box<kFB>(d, fp, fo, fbox);
     __syncthreads();
     const FieldBox fb{fbox, {fo[0], fo[1], fo[2]}, kFB};
     const SpeciesK k = species_constants(d, sp, s);
@@ -921,13 +905,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
         }
     }
     __syncthreads();
-    float* const J[3] = {Jx, Jy, Jz};
-    for (int e = threadIdx.x; e < 3 * kB3; e += kThreads) {
-        const float v = box[e];
-        if (v == 0.0f) continue;
-        const int c = e / kB3, r = e % kB3;
-        deposit_at(J[c], d, o[0] + r / (kB * kB), o[1] + (r / kB) % kB, o[2] + r % kB, v);
-    }
+    flush_current_box<kB>(d, o, box, Jx, Jy, Jz);
 }
 
 // The rare list of fused_tile_direct_kernel: each listed particle runs the

@@ -779,4 +779,67 @@ __device__ inline void fused_block_boxed(const PicDesc& d, const PicSpecies& sp,
     }
 }
 
+// The six field components (plus the external fields) of the kFB^3 nodes from
+// array index fo into the LDS field box (periodic wrap of the indices).  A
+// thread's node offsets are the same for every component, so they are
+// computed once.
+template <int kFB>
+__device__ inline void fill_field_box(const PicDesc& d, const FieldPtrs& fp, const int fo[3], float* fbox) {
+    constexpr int kFV = kFB * kFB * kFB;
+    constexpr int kQ = (kFV + kThreads - 1) / kThreads;
+    int off[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const int r = (int)threadIdx.x + q * kThreads;
+        const int i = pmod(fo[0] + r / (kFB * kFB), d.dims[0]), j = pmod(fo[1] + (r / kFB) % kFB, d.dims[1]);
+        const int k = pmod(fo[2] + r % kFB, d.dims[2]);
+        off[q] = (int)fidx(d, i, j, k);
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        const float* F = c < 3 ? fp.E[c] : fp.B[c - 3];
+        const float* Fe = c < 3 ? fp.Ee[c] : fp.Be[c - 3];
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const int r = (int)threadIdx.x + q * kThreads;
+            if (kFV % kThreads == 0 || r < kFV) {
+                float v = F[off[q]];
+                if (Fe) v += Fe[off[q]];
+                fbox[c * kFV + r] = v;
+            }
+        }
+    }
+}
+
+// Adds the non-zero nodes of the kC-component LDS current box (kB^3 nodes
+// per component from array index o) into J, dropping indices past the array
+// end and wrapping negative ones once (deposit_at); node offsets computed
+// once per thread.
+template <int kB, int kC = 3>
+__device__ inline void flush_current_box(const PicDesc& d, const int o[3], const float* box, float* Jx, float* Jy,
+                                         float* Jz) {
+    constexpr int kB3 = kB * kB * kB;
+    constexpr int kQ = (kB3 + kThreads - 1) / kThreads;
+    float* const J[3] = {Jx, Jy, Jz};
+    int off[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const int r = (int)threadIdx.x + q * kThreads;
+        int i = o[0] + r / (kB * kB), j = o[1] + (r / kB) % kB, k = o[2] + r % kB;
+        const bool in = (kB3 % kThreads == 0 || r < kB3) && drop_index(i, d.dims[0]) && drop_index(j, d.dims[1]) &&
+                        drop_index(k, d.dims[2]);
+        off[q] = in ? (int)fidx(d, i, j, k) : -1;
+    }
+#pragma unroll
+    for (int c = 0; c < kC; ++c) {
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const int r = (int)threadIdx.x + q * kThreads;
+            if (off[q] < 0) continue;
+            const float v = box[c * kB3 + r];
+            if (v != 0.0f) atomic_add_f32(J[c] + off[q], v);
+        }
+    }
+}
+
 }  // namespace

@@ -262,7 +262,7 @@ __global__ __launch_bounds__(kBlock) void tiled_deposit_group_kernel(PicDesc d, 
     constexpr int kU = kRho ? 1 : 3;       // velocity planes held per group (none read for rho)
     constexpr int kChunk = kBlock * kP;
     __shared__ float box[kC * kV];
-    const int b = blockIdx.x;
+    const int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int64_t begin = starts[b], end = starts[b + 1];
     if (begin >= end) return;
     const int t = b % (tiles + 1);
@@ -393,13 +393,7 @@ __global__ __launch_bounds__(kBlock) void tiled_deposit_group_kernel(PicDesc d, 
         cur = nxt;
     }
     __syncthreads();
-    float* const J[3] = {J0, J1, J2};
-    for (int e = threadIdx.x; e < kC * kV; e += kBlock) {
-        const float v = box[e];
-        if (v == 0.0f) continue;
-        const int c = e / kV, r = e % kV;
-        deposit_at(J[c], d, o[0] + r / (kB * kB), o[1] + (r / kB) % kB, o[2] + r % kB, v);
-    }
+    flush_current_box<kB, kC>(d, o, box, J0, J1, J2);
 }
 
 }  // namespace
