@@ -131,6 +131,50 @@ __device__ inline int dpp_row_shr(int v) {
     return __builtin_amdgcn_update_dpp(0, v, 0x110 + kOff, 0xf, 0xf, true);
 }
 
+// Lane i of the wave receives lane i + kOff's value (kOff < 16) inside its
+// aligned row of 16 lanes; lanes whose source leaves the row receive 0.
+template <int kOff>
+__device__ inline int dpp_row_shl(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x100 + kOff, 0xf, 0xf, true);
+}
+
+template <int kOff>
+__device__ inline void row_count_up(int r, int& seg, int& p) {
+    const bool take = seg == 0;
+    const int sb = dpp_row_shr<kOff>(seg), pb = dpp_row_shr<kOff>(p);
+    p += take ? pb : 0;
+    seg |= r >= kOff ? sb : 1;
+}
+
+template <int kOff>
+__device__ inline void row_count_down(int r, int& seg, int& s) {
+    const bool take = seg == 0;
+    const int sb = dpp_row_shl<kOff>(seg), cb = dpp_row_shl<kOff>(s);
+    s += take ? cb : 0;
+    seg |= r + kOff < 16 ? sb : 1;
+}
+
+// Position of each lane in its run of equal keys in consecutive lanes of its
+// aligned row of 16: p = lanes from the run's first lane to this one, s =
+// lanes from this one to the run's last lane (both counting this lane), so the
+// run's last lane in the row is lane + s - 1 and holds the run's length in p.
+// Every lane of the wave must execute it; DPP moves are unconditional.
+__device__ inline void row_run_position(int key, int& p, int& s) {
+    const int r = (int)(threadIdx.x & 15u);
+    const int kprev = dpp_row_shr<1>(key), knext = dpp_row_shl<1>(key);
+    int head = ((r == 0) | (kprev != key)) ? 1 : 0, tail = ((r == 15) | (knext != key)) ? 1 : 0;
+    p = 1;
+    s = 1;
+    row_count_up<1>(r, head, p);
+    row_count_up<2>(r, head, p);
+    row_count_up<4>(r, head, p);
+    row_count_up<8>(r, head, p);
+    row_count_down<1>(r, tail, s);
+    row_count_down<2>(r, tail, s);
+    row_count_down<4>(r, tail, s);
+    row_count_down<8>(r, tail, s);
+}
+
 // Lane i receives lane i + 1's value inside its row of 16 (0 for the last lane).
 __device__ inline int dpp_row_shl1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x101, 0xf, 0xf, true); }
 

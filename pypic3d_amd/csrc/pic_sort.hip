@@ -86,9 +86,12 @@ __global__ void tile_keys_kernel(PicDesc d, PicSpecies sp, PicParticles in, int 
 // A block takes kRebinChunk consecutive slots of the current order and
 // aggregates their keys in an LDS hash table (nearly sorted input has a few
 // dozen distinct keys per chunk, plus the particles that changed cell), so the
-// global atomics are one per distinct key per chunk.  Pass 1 counts the keys
-// and stores them; after the scan, pass 2 reserves each key's positions for
-// the chunk (one atomic per key) and writes the inverse permutation
+// global atomics are one per distinct key per chunk.  Equal keys in
+// consecutive lanes of a 16-lane row are first counted with DPP moves
+// (row_run_position), so only the last lane of such a run touches the table.
+// Pass 1 counts the keys and stores them; after the scan, pass 2 reserves each
+// key's positions for the chunk (one atomic per key, one LDS atomic per run)
+// and writes the inverse permutation
 // perm[dst] = src; the particles are then gathered through perm (reads of
 // nearby slots, coalesced writes).  The order inside one key follows the
 // atomics and is not reproducible; the kernels do not depend on it.
@@ -122,11 +125,16 @@ __global__ void rebin_count_kernel(PicDesc d, PicSpecies sp, PicParticles in, in
     const int64_t b0 = (int64_t)blockIdx.x * kRebinChunk;
     for (int k = 0; k < kRebinItems; ++k) {
         const int64_t i = b0 + (int64_t)k * kBlock + threadIdx.x;
-        if (i >= n) break;
-        const float x[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
-        const uint32_t key = tile_key(d, T, kc, species_of(i, sp.slots, sp.count), in.active[i] != 0, x);
-        keys[i] = key;
-        atomicAdd(&tcnt[hash_insert(tkey, key)], 1u);
+        const bool in_range = i < n;
+        uint32_t key = kEmpty;          // past the end: a run of its own, never inserted
+        if (in_range) {
+            const float x[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
+            key = tile_key(d, T, kc, species_of(i, sp.slots, sp.count), in.active[i] != 0, x);
+            keys[i] = key;
+        }
+        int p, q;
+        row_run_position((int)key, p, q);
+        if (in_range && q == 1) atomicAdd(&tcnt[hash_insert(tkey, key)], (uint32_t)p);
     }
     __syncthreads();
     for (int e = threadIdx.x; e < kHashSize; e += kBlock)
@@ -145,13 +153,26 @@ __global__ void rebin_perm_kernel(const uint32_t* __restrict__ keys, uint32_t* _
     const int64_t b0 = (int64_t)blockIdx.x * kRebinChunk;
     int slot[kRebinItems];
     uint32_t rank[kRebinItems];
+    const int lane = (int)(threadIdx.x & 63u);
     for (int k = 0; k < kRebinItems; ++k) {
         const int64_t i = b0 + (int64_t)k * kBlock + threadIdx.x;
-        slot[k] = -1;
-        if (i < n) {
-            slot[k] = hash_insert(tkey, keys[i]);
-            rank[k] = atomicAdd(&tcnt[slot[k]], 1u);
+        const bool in_range = i < n;
+        const uint32_t key = in_range ? keys[i] : kEmpty;
+        int p, q;
+        row_run_position((int)key, p, q);
+        // the run's last lane inserts the key and reserves the run's p ranks;
+        // the other lanes of the run take its slot and first rank
+        int sl = -1;
+        uint32_t base = 0u;
+        if (in_range && q == 1) {
+            sl = hash_insert(tkey, key);
+            base = atomicAdd(&tcnt[sl], (uint32_t)p);
         }
+        const int src = lane + q - 1;
+        sl = __shfl(sl, src);
+        base = (uint32_t)__shfl((int)base, src);
+        slot[k] = in_range ? sl : -1;
+        rank[k] = base + (uint32_t)(p - 1);
     }
     __syncthreads();
     for (int e = threadIdx.x; e < kHashSize; e += kBlock)
@@ -266,6 +287,49 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, int64_t n,
         else hi = mid;
     }
     starts[b] = (int32_t)lo;
+}
+
+// permute_kernel for four consecutive destinations per thread: one 16-byte
+// load of perm, the 28 gathered loads issued together, 16-byte stores per plane
+// (planes 16-byte aligned, activity bytes 4-byte aligned); a last group of
+// fewer than four takes the per-slot path.
+__global__ void permute4_kernel(PicParticles in, PicParticles out, const int32_t* __restrict__ perm, int64_t n) {
+    const int64_t i0 = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (i0 >= n) return;
+    if (i0 + 4 <= n) {
+        const int4 p4 = *reinterpret_cast<const int4*>(perm + i0);
+        int64_t src[4] = {p4.x, p4.y, p4.z, p4.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (src[j] < 0 || src[j] >= n) src[j] = i0 + j;     // defensive: never read outside the planes
+        float v[6][4];
+        uint32_t act = 0u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                v[a][j] = in.x[a][src[j]];
+                v[3 + a][j] = in.u[a][src[j]];
+            }
+            act |= (uint32_t)in.active[src[j]] << (8 * j);
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            store_vec<4>(out.x[a] + i0, v[a]);
+            store_vec<4>(out.u[a] + i0, v[3 + a]);
+        }
+        *reinterpret_cast<uint32_t*>(out.active + i0) = act;
+        return;
+    }
+    for (int64_t i = i0; i < n; ++i) {
+        int64_t src = perm[i];
+        if (src < 0 || src >= n) src = i;
+        for (int a = 0; a < 3; ++a) {
+            out.x[a][i] = in.x[a][src];
+            out.u[a][i] = in.u[a][src];
+        }
+        out.active[i] = in.active[src];
+    }
 }
 
 __global__ void permute_kernel(PicParticles in, PicParticles out, const int32_t* __restrict__ perm, int64_t n) {
@@ -405,7 +469,13 @@ int pic_tile_rebin(const PicDesc* desc, const PicSpecies* sp, const PicParticles
     PIC_CHECK_LAUNCH();
     if (n > 0) {
         hipLaunchKernelGGL(rebin_perm_kernel, dim3((unsigned)chunks), dim3(kBlock), 0, s, keys, pos, perm, n);
-        hipLaunchKernelGGL(permute_kernel, dim3(grid_1d(n, kBlock)), dim3(kBlock), 0, s, *in, *out, perm, n);
+        bool vec = (((uintptr_t)perm) & 15u) == 0 && (((uintptr_t)out->active) & 3u) == 0;
+        for (int a = 0; a < 3; ++a) vec = vec && ((((uintptr_t)out->x[a]) | ((uintptr_t)out->u[a])) & 15u) == 0;
+        if (vec)
+            hipLaunchKernelGGL(permute4_kernel, dim3(grid_1d((n + 3) / 4, kBlock)), dim3(kBlock), 0, s, *in, *out,
+                               perm, n);
+        else
+            hipLaunchKernelGGL(permute_kernel, dim3(grid_1d(n, kBlock)), dim3(kBlock), 0, s, *in, *out, perm, n);
         PIC_CHECK_LAUNCH();
     }
     return 0;
