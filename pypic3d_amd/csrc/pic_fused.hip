@@ -321,7 +321,7 @@ __device__ inline void crossing_deposit_box(const SpeciesK& k, const CrossAxis& 
 template <int kT, int kP, int kMinBlocks>
 __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
     PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, PicParticles out, float* Jx, float* Jy, float* Jz,
-    const int32_t* __restrict__ starts, TileGeom tg, ExactGrid eg, int32_t* __restrict__ rare, int vec_ok) {
+    const int32_t* __restrict__ starts, TileGeom tg, ExactGrid eg, int32_t* __restrict__ rare, int vec_ok, SpeciesTable st) {
     constexpr int kB = kT + 4;
     constexpr int kB3 = kB * kB * kB;
     constexpr int kFB = kT + 4;
@@ -355,7 +355,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
     fill_field_box<kFB>(d, fp, fo, fbox);
     const BoxSink sink{{Jx, Jy, Jz}, box, {o[0], o[1], o[2]}, kB};
     const FieldBox fb{fbox, {fo[0], fo[1], fo[2]}, kFB};
-    const SpeciesK k = species_constants(d, sp, s);
+    const SpeciesK k = st.k[s];
     const float fdt = (float)d.dt;
 
     for (int64_t base = begin & ~(int64_t)(kP - 1); base < end; base += kChunk) {
@@ -537,7 +537,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
                 const bool same = have && anc[0] == h0 && anc[1] == h1 && anc[2] == h2;
                 if (have && !same) flush_cell<kB>(d, sink, h0, h1, h2, acc);
 #pragma unroll
-                for (int m = 0; m < 12; ++m) acc[m] = same ? acc[m] + val[m] : val[m];
+                for (int m = 0; m < 12; ++m) acc[m] = (same ? acc[m] : 0.0f) + val[m];
                 h0 = anc[0];
                 h1 = anc[1];
                 h2 = anc[2];
@@ -646,7 +646,7 @@ __global__ void zero_count_kernel(int32_t* c) { *c = 0; }
 template <int kT, int kMinBlocks, bool kVec>
 __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel(
     PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, PicParticles out, float* Jx, float* Jy, float* Jz,
-    const int32_t* __restrict__ starts, TileGeom tg, ExactGrid eg, int32_t* __restrict__ rare, int vec_ok) {
+    const int32_t* __restrict__ starts, TileGeom tg, ExactGrid eg, int32_t* __restrict__ rare, int vec_ok, SpeciesTable st) {
     constexpr int kP = 4;
     constexpr int kB = kT + 5;
     constexpr int kB3 = kB * kB * kB;
@@ -671,7 +671,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
     fill_field_box<kFB>(d, fp, fo, fbox);
     __syncthreads();
     const FieldBox fb{fbox, {fo[0], fo[1], fo[2]}, kFB};
-    const SpeciesK k = species_constants(d, sp, s);
+    const SpeciesK k = st.k[s];
     const double hdt = 0.5 * d.dt;
     auto flush = [&](int h0, int h1, int h2, const float* acc) {
         float* p = box + ((h0 - o[0]) * kB + (h1 - o[1])) * kB + (h2 - o[2]);
@@ -838,7 +838,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
                 const bool same = have && an[0] == h0 && an[1] == h1 && an[2] == h2;
                 if (have && !same) flush(h0, h1, h2, acc);
 #pragma unroll
-                for (int m = 0; m < 24; ++m) acc[m] = same ? acc[m] + val[m] : val[m];
+                for (int m = 0; m < 24; ++m) acc[m] = (same ? acc[m] : 0.0f) + val[m];
                 h0 = an[0];
                 h1 = an[1];
                 h2 = an[2];
@@ -1052,9 +1052,10 @@ int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const
                     ((uintptr_t)in->u[a])) & 15u) == 0;
     vec_ok &= ((((uintptr_t)out->active) | ((uintptr_t)in->active)) & 3u) == 0;
     hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, workspace);
+    const SpeciesTable stab = species_table(*desc, *sp);
 #define PIC_DIRECT_LAUNCH(T, MB, V)                                                                               \
     hipLaunchKernelGGL((fused_tile_direct_kernel<T, MB, V>), dim3((unsigned)blocks), dim3(kThreads), 0, st, *desc,   \
-                       *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace, vec_ok)
+                       *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace, vec_ok, stab)
     if (tile == 8) {
         if (min_blocks == 4) PIC_DIRECT_LAUNCH(8, 4, false);
         else PIC_DIRECT_LAUNCH(8, 1, true);
@@ -1118,9 +1119,10 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
                         ((uintptr_t)in->u[a])) & 15u) == 0;
         vec_ok &= ((((uintptr_t)out->active) | ((uintptr_t)in->active)) & 3u) == 0;
         hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, workspace);
+        const SpeciesTable stab = species_table(*desc, *sp);
 #define PIC_EXACT_LAUNCH(T, P, MB)                                                                                  \
         hipLaunchKernelGGL((fused_tile_exact_kernel<T, P, MB>), dim3((unsigned)blocks), dim3(kThreads), 0, st,      \
-                           *desc, *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace, vec_ok)
+                           *desc, *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace, vec_ok, stab)
 #define PIC_EXACT_MB(T, P)                                  \
         do {                                                \
             if (min_blocks == 3) PIC_EXACT_LAUNCH(T, P, 3); \

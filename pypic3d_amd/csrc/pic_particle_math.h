@@ -78,7 +78,7 @@ struct SpeciesK {
     uint32_t ux, uu;
 };
 
-__device__ inline SpeciesK species_constants(const PicDesc& d, const PicSpecies& sp, int s) {
+__host__ __device__ inline SpeciesK species_constants(const PicDesc& d, const PicSpecies& sp, int s) {
     SpeciesK k;
     k.qw = sp.charge[s] * sp.weight[s];
     k.h = (float)(sp.charge[s] * d.dt / (2.0 * sp.mass[s]));
@@ -92,6 +92,19 @@ __device__ inline SpeciesK species_constants(const PicDesc& d, const PicSpecies&
         k.uu |= (sp.update_u[s][a] ? 1u : 0u) << a;
     }
     return k;
+}
+
+// Species constants of every species, computed once on the host and passed by
+// value, so a block reads its species' constants with scalar loads instead of
+// evaluating the FP64 divisions in every lane.
+struct SpeciesTable {
+    SpeciesK k[PIC_MAX_SPECIES];
+};
+
+inline SpeciesTable species_table(const PicDesc& d, const PicSpecies& sp) {
+    SpeciesTable t{};
+    for (int s = 0; s < sp.count && s < PIC_MAX_SPECIES; ++s) t.k[s] = species_constants(d, sp, s);
+    return t;
 }
 
 // Six staggered components (particle_push.py:63-68): Ex (v,c,c), Ey (c,v,c),
