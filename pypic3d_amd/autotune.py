@@ -47,7 +47,7 @@ def tune_fused(desc, sp, pp_in, E3, B3, Ee, Be, pp_out, J3, reps=3, variants=Non
     return best, times
 
 
-def tune_tiled(desc, sp, pp_in, E3, B3, Ee, Be, pp_out, J3, tile, starts, workspace, reps=3, rounds=(2, 4),
+def tune_tiled(desc, sp, pp_in, E3, B3, Ee, Be, pp_out, J3, tile, starts, workspace, reps=3, rounds=(2, 4, 8),
                min_blocks=(1, 3, 4)):
     """Launch shape of the tile-ordered kernel: ((rounds, min_blocks), {"rounds/min_blocks": mean ms})."""
     s = N.stream_handle()

@@ -318,7 +318,7 @@ __device__ inline void crossing_deposit_box(const SpeciesK& k, const CrossAxis& 
     crossing_component_box<2, kB>(k.pre[2], cz, cx, cy, p);
 }
 
-template <int kT, int kP, int kMinBlocks>
+template <int kT, int kP, int kMinBlocks, int kG>
 __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
     PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, PicParticles out, float* Jx, float* Jy, float* Jz,
     const int32_t* __restrict__ starts, TileGeom tg, ExactGrid eg, int32_t* __restrict__ rare, int vec_ok, SpeciesTable st) {
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
     constexpr int kB3 = kB * kB * kB;
     constexpr int kFB = kT + 4;
     constexpr int kFV = kFB * kFB * kFB;
-    constexpr int kChunk = kThreads * kP;
+    constexpr int kChunk = kThreads * kP * kG;     // a thread owns kG * kP consecutive slots
     __shared__ CrossQueue sh;
     __shared__ float box[3 * kB3];
     __shared__ float fbox[6 * kFV];
@@ -361,13 +361,15 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
     for (int64_t base = begin & ~(int64_t)(kP - 1); base < end; base += kChunk) {
         if (threadIdx.x == 0) sh.q_n = 0;
         __syncthreads();    // also orders the box fills before the first deposits
-        const int64_t s0 = base + (int64_t)kP * threadIdx.x;
-        const bool full = vec_ok && s0 >= begin && s0 + kP <= end;
         float acc[12];
 #pragma unroll
         for (int m = 0; m < 12; ++m) acc[m] = 0.0f;
         int h0 = 0, h1 = 0, h2 = 0;
         bool have = false;
+#pragma unroll 1
+        for (int g = 0; g < kG; ++g) {
+        const int64_t s0 = base + (int64_t)kP * (kG * (int)threadIdx.x + g);
+        const bool full = vec_ok && s0 >= begin && s0 + kP <= end;
         // The thread's kP slots are loaded at once, with one kP-wide vector load
         // per plane when the group lies inside the range (the lanes of a wave
         // then read contiguous memory, and all loads share one round trip).
@@ -484,7 +486,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
                 // and a frame inside the box, else the rare list; the crossing pass
                 // rebuilds the frame from (step, ro, f, old anchor)
                 bool ok = short_step;
-                int packed = kP * threadIdx.x + j;
+                int packed = 0;
                 float fro[3], ff[3];
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
@@ -587,8 +589,9 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
                 out.active[i] = (uint8_t)((oact >> (8 * j)) & 0xffu);
             }
         }
+        }   // slot groups
         {
-            // the group's last cell: equal cells of neighbouring lanes are summed
+            // the thread's last cell: equal cells of neighbouring lanes are summed
             // first, so one lane per cell and 16-lane row adds into the box (a
             // cell outside the box keeps a key of its own)
             const int bi = h0 - o[0], bj = h1 - o[1], bk = h2 - o[2];
@@ -1082,7 +1085,8 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
                                    const int32_t* starts, int32_t rounds, int32_t min_blocks, int32_t* workspace,
                                    void* stream) {
     PIC_REQUIRE(desc_ok(desc) && particles_ok(sp, in) && particles_ok(sp, out) && E && B && J && starts);
-    PIC_REQUIRE(fused_kind(*desc) == 3 && (tile == 4 || tile == 8) && (rounds == 2 || rounds == 4));
+    PIC_REQUIRE(fused_kind(*desc) == 3 && (tile == 4 || tile == 8) &&
+                (rounds == 2 || rounds == 4 || rounds == 8));
     PIC_REQUIRE(min_blocks == 1 || min_blocks == 3 || min_blocks == 4);
     const bool empty = (int64_t)sp->count * sp->slots == 0;
     for (int a = 0; a < 3; ++a)
@@ -1120,21 +1124,24 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
         vec_ok &= ((((uintptr_t)out->active) | ((uintptr_t)in->active)) & 3u) == 0;
         hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, workspace);
         const SpeciesTable stab = species_table(*desc, *sp);
-#define PIC_EXACT_LAUNCH(T, P, MB)                                                                                  \
-        hipLaunchKernelGGL((fused_tile_exact_kernel<T, P, MB>), dim3((unsigned)blocks), dim3(kThreads), 0, st,      \
+#define PIC_EXACT_LAUNCH(T, P, MB, G)                                                                               \
+        hipLaunchKernelGGL((fused_tile_exact_kernel<T, P, MB, G>), dim3((unsigned)blocks), dim3(kThreads), 0, st,   \
                            *desc, *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace, vec_ok, stab)
-#define PIC_EXACT_MB(T, P)                                  \
-        do {                                                \
-            if (min_blocks == 3) PIC_EXACT_LAUNCH(T, P, 3); \
-            else if (min_blocks == 4) PIC_EXACT_LAUNCH(T, P, 4); \
-            else PIC_EXACT_LAUNCH(T, P, 1);                 \
+#define PIC_EXACT_MB(T, P, G)                                  \
+        do {                                                   \
+            if (min_blocks == 3) PIC_EXACT_LAUNCH(T, P, 3, G); \
+            else if (min_blocks == 4) PIC_EXACT_LAUNCH(T, P, 4, G); \
+            else PIC_EXACT_LAUNCH(T, P, 1, G);                 \
         } while (0)
+        // rounds: slots per thread (8: two groups of four)
         if (tile == 8) {
-            if (rounds == 2) PIC_EXACT_MB(8, 2);
-            else PIC_EXACT_MB(8, 4);
+            if (rounds == 2) PIC_EXACT_MB(8, 2, 1);
+            else if (rounds == 4) PIC_EXACT_MB(8, 4, 1);
+            else PIC_EXACT_MB(8, 4, 2);
         } else {
-            if (rounds == 2) PIC_EXACT_MB(4, 2);
-            else PIC_EXACT_MB(4, 4);
+            if (rounds == 2) PIC_EXACT_MB(4, 2, 1);
+            else if (rounds == 4) PIC_EXACT_MB(4, 4, 1);
+            else PIC_EXACT_MB(4, 4, 2);
         }
 #undef PIC_EXACT_MB
 #undef PIC_EXACT_LAUNCH

@@ -182,7 +182,7 @@ def test_fused_step_on_cell_sorted_dense_plasma_matches_oracle(N, ppc, pusher):
 
 @pytest.mark.parametrize("tile,rounds,min_blocks,dep", [(None, 2, 1, "esirkepov"), (8, 2, 3, "esirkepov"),
                                                        (4, 2, 1, "esirkepov"), (8, 4, 1, "esirkepov"),
-                                                       (4, 4, 4, "esirkepov"), (8, 4, 1, "direct"),
+                                                       (4, 4, 4, "esirkepov"), (4, 8, 4, "esirkepov"), (8, 4, 1, "direct"),
                                                        (4, 4, 1, "direct")])
 def test_runner_matches_time_loop_and_sorting_is_transparent(tile, rounds, min_blocks, dep):
     """Slot-ordered and tile-ordered runners (re-sorting every 2 steps, the

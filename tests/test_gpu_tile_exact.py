@@ -51,7 +51,7 @@ def _launch(r, variant):
             r.act[dst].clone(), int(r.tile_ws[0]))
 
 
-@pytest.mark.parametrize("tile,rounds,min_blocks", [(4, 4, 4), (4, 2, 1), (8, 4, 3)])
+@pytest.mark.parametrize("tile,rounds,min_blocks", [(4, 4, 4), (4, 2, 1), (8, 4, 3), (4, 8, 4), (8, 8, 1)])
 def test_exact_tile_kernel_agrees_with_strided_tile_kernel(tile, rounds, min_blocks):
     """Same sorted input, one launch each: identical particles, J equal up to
     FP32 summation order; some particles have drifted out of their tile or
