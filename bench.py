@@ -176,7 +176,7 @@ def extras(n, ppc, steps, seed):
         static, dyn, species = workloads.config_warm_3d(n=n, ppc=ppc, vth=0.05, current_deposition="j_from_rhov")
         static = static._replace(current_filter=filt)
         parts, cfg = workloads.plasma(static, dyn, species, seed=seed)
-        r = ElectrodynamicRunner(parts, cfg, workloads.empty_fields(static, dyn), static, dyn, sort_every=10, tile=8)
+        r = ElectrodynamicRunner(parts, cfg, workloads.empty_fields(static, dyn), static, dyn, sort_every=6, tile=4)
         del parts
         ms = _timed(r.step, steps)
         npart = r.S * r.P

@@ -664,12 +664,20 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
     const int t = b % (tg.tiles + 1);
     const int s = b / (tg.tiles + 1);
     int o[3] = {0, 0, 0}, fo[3] = {0, 0, 0};
+    // interior: a fast-path particle (anchors inside the field box, cells
+    // c0 - 2 .. c0 + T + 1) that moves less than a cell per step stays inside
+    // [-W/2, W/2) on every axis, so its particle boundaries do nothing
+    bool interior = t < tg.tiles;
     if (t < tg.tiles) {
         const int tx = t / (tg.nt[1] * tg.nt[2]), ty = (t / tg.nt[2]) % tg.nt[1], tz = t % tg.nt[2];
         const int c0[3] = {tx * kT, ty * kT, tz * kT};
 #pragma unroll
-        for (int a = 0; a < 3; ++a) o[a] = fo[a] = c0[a] + d.g - 2;
+        for (int a = 0; a < 3; ++a) {
+            o[a] = fo[a] = c0[a] + d.g - 2;
+            interior = interior && c0[a] >= 3 && c0[a] + kT + 3 <= d.n[a];
+        }
     }
+    const float fdt = (float)d.dt;
     for (int e = threadIdx.x; e < 3 * kB3; e += kThreads) box[e] = 0.0f;
     fill_field_box<kFB>(d, fp, fo, fbox);
     __syncthreads();
@@ -792,13 +800,18 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
                 if ((k.uu >> a) & 1u) v[a] = nv[a];
             // ---- x += v dt/2, boundaries (evolve.py:48-52)
             bool act = true;
+            bool skip_bc = interior;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) skip_bc = skip_bc && fabsf(v[a]) * fdt < 0.99f * eg.sp_c[a];
             float xh[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 double t = ((k.ux >> a) & 1u) ? (double)(float)step_pos((double)xf[a], (double)v[a], hdt)
                                               : (double)xf[a];
-                if (d.pbc[a] == 0) periodic_wrap(d.wind[a], t);
-                else apply_axis_boundary(d.pbc[a], d.wind[a], t, v[a], act);
+                if (!skip_bc) {
+                    if (d.pbc[a] == 0) periodic_wrap(d.wind[a], t);
+                    else apply_axis_boundary(d.pbc[a], d.wind[a], t, v[a], act);
+                }
                 xh[a] = (float)t;
             }
             if (act) {
@@ -851,8 +864,10 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
                 for (int a = 0; a < 3; ++a) {
                     double t2 = ((k.ux >> a) & 1u) ? (double)(float)step_pos((double)xh[a], (double)v[a], hdt)
                                                    : (double)xh[a];
-                    if (d.pbc[a] == 0) periodic_wrap(d.wind[a], t2);
-                    else apply_axis_boundary(d.pbc[a], d.wind[a], t2, v[a], act);
+                    if (!skip_bc) {
+                        if (d.pbc[a] == 0) periodic_wrap(d.wind[a], t2);
+                        else apply_axis_boundary(d.pbc[a], d.wind[a], t2, v[a], act);
+                    }
                     ox[a] = (float)t2;
                     ou[a] = v[a];
                 }
