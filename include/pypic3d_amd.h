@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define PIC_ABI_VERSION 9
+#define PIC_ABI_VERSION 10
 #define PIC_MAX_SPECIES 16
 
 /* Scalar run description: StaticParameters + DynamicParameters fields the hot
@@ -110,6 +110,11 @@ int pic_yee_update_e(const PicDesc* desc, const float* const E_in[3], const floa
 typedef struct PicPml {
     float* mem[6];
     const float* sigma[3];
+    /* optional (NULL: unused): sigma_line[a][i] = sigma[a] at index i of axis
+     * a (dims[a] entries) for a profile that varies along its own axis only
+     * over the interior, as PML.py's are; the kernels then read the line
+     * instead of the field-shaped profile (results are identical). */
+    const float* sigma_line[3];
 } PicPml;
 /* pic_yee_update_b / _e with apply_tiled_pml_to_b_curl / _e_curl
  * (PML.py:270-359; first_order_yee.py:54-59, 133-138) applied to the

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libpypic3d_hip.so")
 MAX_SPECIES = 16
 PUSHERS = {"boris": 0, "higuera_cary": 1}
 FUSED_VARIANTS = 6
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 c_int = ctypes.c_int
 c_i32 = ctypes.c_int32
@@ -51,7 +51,7 @@ class PicParticles(ctypes.Structure):
 
 
 class PicPml(ctypes.Structure):
-    _fields_ = [("mem", c_vp * 6), ("sigma", c_vp * 3)]
+    _fields_ = [("mem", c_vp * 6), ("sigma", c_vp * 3), ("sigma_line", c_vp * 3)]
 
 
 DP = ctypes.POINTER(PicDesc)

@@ -149,7 +149,7 @@ def _apply(derivatives, static_parameters, dynamic_parameters, pml_state, which)
     dv = [t.reshape(n).to(torch.float32).contiguous().clone() for t in derivatives]
     from ..solvers.first_order_yee import pml_struct
     ptrs = (N.c_vp * 6)(*[t.data_ptr() for t in dv])
-    N.call("pic_pml_stretch", desc, ptrs, ctypes.byref(pml_struct(mem3, prof3)), N.stream_handle())
+    N.call("pic_pml_stretch", desc, ptrs, ctypes.byref(pml_struct(mem3, prof3, desc=desc)), N.stream_handle())
     shape = derivatives[0].shape
     dv = [t.reshape(shape) for t in dv]
     curls = (dv[0] - dv[1], dv[2] - dv[3], dv[4] - dv[5])

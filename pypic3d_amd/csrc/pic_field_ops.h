@@ -185,17 +185,23 @@ __device__ inline void yee_e_point(const PicDesc& d, const YeeConstT<T>& y, cons
 // m' = b m + (b - 1) d and the derivative d + m'.  Derivative order (and memory
 // order): d?z/dy, d?y/dz, d?x/dz, d?z/dx, d?y/dx, d?x/dy; curl = (d0 - d1,
 // d2 - d3, d4 - d5).  mem[] are interior (Nx, Ny, Nz) arrays updated in place.
+// line[a] (optional) holds profile a along its own axis, for profiles that do
+// not vary along the other two: one cached value per point instead of three
+// reads of field-shaped arrays.
 struct PmlArgs {
     float* mem[6];
     const float* sigma[3];
+    const float* line[3];
     float dt;
 };
 
 __device__ inline void pml_stretch6(const PicDesc& d, const PmlArgs& pml, int i, int j, int k, float dv[6]) {
     const int64_t c = fidx(d, i, j, k);
     const int64_t mi = ((int64_t)(i - d.g) * d.n[1] + (j - d.g)) * d.n[2] + (k - d.g);
-    const float b[3] = {expf(-pml.sigma[0][c] * pml.dt), expf(-pml.sigma[1][c] * pml.dt),
-                        expf(-pml.sigma[2][c] * pml.dt)};
+    const int idx[3] = {i, j, k};
+    float b[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) b[a] = expf(-(pml.line[a] ? pml.line[a][idx[a]] : pml.sigma[a][c]) * pml.dt);
     constexpr int kAxis[6] = {1, 2, 2, 0, 0, 1};
 #pragma unroll
     for (int q = 0; q < 6; ++q) {

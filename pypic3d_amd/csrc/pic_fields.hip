@@ -413,7 +413,10 @@ static bool pml_ok(const PicPml* pml) {
 static PmlArgs pml_args(const PicDesc* desc, const PicPml* pml) {
     PmlArgs a;
     for (int q = 0; q < 6; ++q) a.mem[q] = pml->mem[q];
-    for (int c = 0; c < 3; ++c) a.sigma[c] = pml->sigma[c];
+    for (int c = 0; c < 3; ++c) {
+        a.sigma[c] = pml->sigma[c];
+        a.line[c] = pml->sigma_line[c];
+    }
     a.dt = (float)desc->dt;          // the stretch uses the full step in both updates (PML.py:291, 337)
     return a;
 }

@@ -13,6 +13,7 @@ memory is returned in a new state.
 from __future__ import annotations
 
 import ctypes
+import weakref
 
 import torch
 
@@ -42,13 +43,55 @@ def pml_memory3(desc, mem):
     return out
 
 
-def pml_struct(mem6, sigma3):
-    """PicPml over six (Nx, Ny, Nz) memory tensors and three field-shaped profiles."""
+_SIGMA_LINES = {}
+
+
+def sigma_lines(desc, sigma3):
+    """Profile a along its own axis (dims[a] float32 values, contiguous) for
+    each of the three field-shaped profiles, when every profile is constant
+    along the other two axes over the interior (PML.py's are); else None.
+    Checked once per profile tensor and version (an in-place change of a
+    profile, through any view, bumps its version and triggers a new check)."""
+    key = tuple(id(t) for t in sigma3)
+    hit = _SIGMA_LINES.get(key)
+    if hit is not None:
+        refs, versions, lines = hit
+        if all(r() is t and t._version == v for r, t, v in zip(refs, sigma3, versions)):
+            return lines
+    g = int(desc.g)
+    dims = tuple(int(v) for v in desc.dims)
+    n = tuple(int(v) for v in desc.n)
+    lines = []
+    for a, t in enumerate(sigma3):
+        f = t.reshape(dims)
+        at = [g, g, g]
+        at[a] = slice(None)
+        line = f[tuple(at)].contiguous()
+        shape = [1, 1, 1]
+        shape[a] = n[a]
+        want = line[g:g + n[a]].reshape(shape).expand(n)
+        if not torch.equal(f[g:g + n[0], g:g + n[1], g:g + n[2]], want):
+            lines = None
+            break
+        lines.append(line)
+    _SIGMA_LINES[key] = (tuple(weakref.ref(t) for t in sigma3), tuple(t._version for t in sigma3), lines)
+    return lines
+
+
+def pml_struct(mem6, sigma3, desc=None):
+    """PicPml over six (Nx, Ny, Nz) memory tensors and three field-shaped
+    profiles (with their axis lines when desc is given and the profiles are
+    separable).  The returned struct keeps the line tensors alive."""
     s = N.PicPml()
     for q in range(6):
         s.mem[q] = mem6[q].data_ptr()
     for a in range(3):
         s.sigma[a] = sigma3[a].data_ptr()
+    lines = sigma_lines(desc, sigma3) if desc is not None else None
+    if lines is not None:
+        for a in range(3):
+            s.sigma_line[a] = lines[a].data_ptr()
+        s._keep = lines
     return s
 
 
@@ -62,8 +105,8 @@ def yee_b(desc, E3, B3, out3, do_filter, stream, scratch=None, pml=None):
     if pml is None:
         N.call(N.op("pic_yee_update_b", f64), desc, N.p3(E3), N.p3(B3), N.p3(out3), stream)
     else:
-        N.call("pic_yee_update_b_pml", desc, N.p3(E3), N.p3(B3), N.p3(out3), ctypes.byref(pml_struct(*pml)),
-               stream)
+        N.call("pic_yee_update_b_pml", desc, N.p3(E3), N.p3(B3), N.p3(out3),
+               ctypes.byref(pml_struct(*pml, desc=desc)), stream)
     if do_filter and desc.alpha != 1.0:
         tmp = scratch if scratch is not None else [c.clone() for c in out3]
         for c, t in zip(out3, tmp):
@@ -83,7 +126,7 @@ def yee_e(desc, E3, B3, J3, out3, stream, scratch=None, pml=None):
         N.call(N.op("pic_yee_update_e", f64), desc, N.p3(E3), N.p3(B3), N.p3(J3), N.p3(out3), stream)
     else:
         N.call("pic_yee_update_e_pml", desc, N.p3(E3), N.p3(B3), N.p3(J3), N.p3(out3),
-               ctypes.byref(pml_struct(*pml)), stream)
+               ctypes.byref(pml_struct(*pml, desc=desc)), stream)
     if desc.alpha != 1.0:
         tmp = scratch if scratch is not None else [c.clone() for c in out3]
         for c, t in zip(out3, tmp):

@@ -141,3 +141,42 @@ def test_runner_with_pml_matches_the_functional_step():
     for which in (0, 1):
         for q in range(6):
             assert rel_err(host(r_fields[6][which][q]), host(ref_fields[6][which][q])) <= 1e-5
+
+
+def test_axis_lines_give_the_same_bits_as_field_shaped_profiles():
+    """PML.py's profiles vary along their own axis only, so the kernels read
+    one line per axis (PicPml.sigma_line); the results equal, bit for bit,
+    those of the field-shaped profiles, and a profile that varies along
+    another axis is detected and keeps the field-shaped path."""
+    import ctypes
+    from pypic3d_amd import _native as N
+    from pypic3d_amd._tensors import make_desc
+    from pypic3d_amd.solvers.first_order_yee import pml_struct, sigma_lines
+    static, dyn, p, state, mem, oprof = _setup(seed=7)
+    desc = make_desc(static, dyn)
+    prof = state[2]
+    assert sigma_lines(desc, prof) is not None
+    E = [gpu_field(random_field(p, 40 + c)).reshape(p.shape) for c in range(3)]
+    B = [gpu_field(random_field(p, 50 + c)).reshape(p.shape) for c in range(3)]
+
+    def run(with_lines, profiles):
+        m = [t.reshape(p.N).clone() for t in state[1]]
+        out = [torch.empty_like(c) for c in B]
+        s = pml_struct(m, profiles, desc=desc if with_lines else None)
+        assert (s.sigma_line[0] is not None) == with_lines
+        N.call("pic_yee_update_b_pml", desc, N.p3(E), N.p3(B), N.p3(out), ctypes.byref(s), N.stream_handle())
+        torch.cuda.synchronize()
+        return out, m
+
+    o1, m1 = run(True, prof)
+    o0, m0 = run(False, prof)
+    for a, b in zip(o1 + m1, o0 + m0):
+        assert torch.equal(a, b)
+    bent = tuple(t.clone() for t in prof)
+    g = int(desc.g)
+    at = (g + 1, g + 2, g + 1)
+    keep = bent[0].view(p.shape)[at].clone()
+    bent[0].view(p.shape)[at] = keep + 1.0                  # the x profile now varies along y
+    assert sigma_lines(desc, bent) is None
+    bent[0].view(p.shape)[at] = keep                        # restored in place: checked again
+    assert sigma_lines(desc, bent) is not None
