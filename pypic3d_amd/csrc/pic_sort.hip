@@ -294,7 +294,10 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, int64_t n,
 // (planes 16-byte aligned, activity bytes 4-byte aligned); a last group of
 // fewer than four takes the per-slot path.
 __global__ void permute4_kernel(PicParticles in, PicParticles out, const int32_t* __restrict__ perm, int64_t n) {
-    const int64_t i0 = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    // XCD-contiguous destination ranges: a particle that moved to a nearby
+    // cell is read from slots the same XCD has just streamed through its L2
+    const int64_t blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int64_t i0 = 4 * (blk * blockDim.x + threadIdx.x);
     if (i0 >= n) return;
     if (i0 + 4 <= n) {
         const int4 p4 = *reinterpret_cast<const int4*>(perm + i0);
