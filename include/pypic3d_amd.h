@@ -205,11 +205,11 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
  * pic_deposit_direct, pic_advance_positions, pic_particle_boundaries up to
  * FP32 rounding (an ulp in a velocity where the compiler fuses the push's
  * multiply-adds differently) and summation order in J.  tile 4 or 8;
- * min_blocks 1 (each thread's four slots stored with 16-byte stores) or 4
- * (slots stored one by one for 4 resident blocks per CU at tile 4: about 4 %
- * faster at 256^3 x 32 ppc but 2.1x the HBM traffic, since the 4-byte
- * stores at a 16-byte lane stride reach HBM as partial lines); workspace as
- * for pic_fused_push_esirkepov_tiled. */
+ * min_blocks 1 (each thread's four slots loaded and stored with 16-byte
+ * accesses) or 4 (tile 4: two slots per thread with 8-byte accesses, which
+ * fits 128 VGPRs for 4 resident blocks per CU, about 15 % faster at 256^3 x
+ * 32 ppc; tile 8: slots stored one by one); workspace as for
+ * pic_fused_push_esirkepov_tiled. */
 int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
                                 const float* const E[3], const float* const B[3], const float* const E_ext[3],
                                 const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,

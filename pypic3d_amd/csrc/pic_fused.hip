@@ -646,11 +646,10 @@ __global__ void zero_count_kernel(int32_t* c) { *c = 0; }
 // a thread's four slots are stored with one 16-byte store per plane at the
 // end (24 more registers); otherwise each slot is stored as it finishes,
 // which keeps the kernel at 128 VGPRs for 4 waves per SIMD.
-template <int kT, int kMinBlocks, bool kVec>
+template <int kT, int kMinBlocks, bool kVec, int kP = 4>
 __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel(
     PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, PicParticles out, float* Jx, float* Jy, float* Jz,
     const int32_t* __restrict__ starts, TileGeom tg, ExactGrid eg, int32_t* __restrict__ rare, int vec_ok, SpeciesTable st) {
-    constexpr int kP = 4;
     constexpr int kB = kT + 5;
     constexpr int kB3 = kB * kB * kB;
     constexpr int kFB = kT + 4;
@@ -713,7 +712,8 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
                     load_vec<kP>(in.x[a] + s0, OX[a]);
                     load_vec<kP>(in.u[a] + s0, OU[a]);
                 }
-                pact = *reinterpret_cast<const uint32_t*>(in.active + s0);
+                if constexpr (kP == 4) pact = *reinterpret_cast<const uint32_t*>(in.active + s0);
+                else pact = *reinterpret_cast<const uint16_t*>(in.active + s0);
             } else {
 #pragma unroll
                 for (int j = 0; j < kP; ++j) {
@@ -907,7 +907,8 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
                 store_vec<kP>(out.x[a] + s0, OX[a]);
                 store_vec<kP>(out.u[a] + s0, OU[a]);
             }
-            *reinterpret_cast<uint32_t*>(out.active + s0) = oact;
+            if constexpr (kP == 4) *reinterpret_cast<uint32_t*>(out.active + s0) = oact;
+            else *reinterpret_cast<uint16_t*>(out.active + s0) = (uint16_t)oact;
         } else {
 #pragma unroll
             for (int j = 0; j < kP; ++j) {
@@ -1071,6 +1072,9 @@ int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const
     vec_ok &= ((((uintptr_t)out->active) | ((uintptr_t)in->active)) & 3u) == 0;
     hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, workspace);
     const SpeciesTable stab = species_table(*desc, *sp);
+#define PIC_DIRECT_LAUNCH2(T, MB, V, P)                                                                           \
+    hipLaunchKernelGGL((fused_tile_direct_kernel<T, MB, V, P>), dim3((unsigned)blocks), dim3(kThreads), 0, st, *desc, \
+                       *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace, vec_ok, stab)
 #define PIC_DIRECT_LAUNCH(T, MB, V)                                                                               \
     hipLaunchKernelGGL((fused_tile_direct_kernel<T, MB, V>), dim3((unsigned)blocks), dim3(kThreads), 0, st, *desc,   \
                        *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace, vec_ok, stab)
@@ -1078,7 +1082,7 @@ int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const
         if (min_blocks == 4) PIC_DIRECT_LAUNCH(8, 4, false);
         else PIC_DIRECT_LAUNCH(8, 1, true);
     } else {
-        if (min_blocks == 4) PIC_DIRECT_LAUNCH(4, 4, false);
+        if (min_blocks == 4) PIC_DIRECT_LAUNCH2(4, 4, true, 2);
         else PIC_DIRECT_LAUNCH(4, 1, true);
     }
 #undef PIC_DIRECT_LAUNCH

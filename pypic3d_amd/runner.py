@@ -82,6 +82,9 @@ class ElectrodynamicRunner:
         self.keys = None
         self.sorts_done = 0
         self.tile, self.rounds, self.min_blocks, self.starts = None, int(rounds), int(min_blocks), None
+        # launch shape of the fused direct kernel: 4 resident blocks per CU
+        # (two slots per thread) at tile 4, one at tile 8
+        self.direct_min_blocks = 1
         self.tile_ws, self.direct_fused = None, False
         if tile is not None:
             self.set_tile(tile)
@@ -135,7 +138,8 @@ class ElectrodynamicRunner:
                 N.call("pic_fill_f32", N.ptr(c), c.numel(), 0.0, s)
             ev = self._mark("particles")
             N.call("pic_fused_push_direct_tiled", d, self.sp, self._pp(src), N.p3(E), N.p3(self.B), self.Ee, self.Be,
-                   self._pp(dst), N.p3(self.J), self.tile, N.ptr(self.starts), 1, N.ptr(self.tile_ws), s)
+                   self._pp(dst), N.p3(self.J), self.tile, N.ptr(self.starts), self.direct_min_blocks,
+                   N.ptr(self.tile_ws), s)
             if ev:
                 ev[1].record()
             ev = self._mark("current_fold")
@@ -217,6 +221,7 @@ class ElectrodynamicRunner:
         if tile not in (4, 8):
             raise ValueError("tile must be 4 or 8")
         self.tile = int(tile)
+        self.direct_min_blocks = 4 if self.tile == 4 else 1
         # direct branch: one fused tile kernel on exact 3-D first-order grids with cells equal to the node
         # spacing, the separate push / advance / boundary / tiled-deposit kernels otherwise
         self.direct_fused = (not self.esirkepov and d.shape_factor == 1 and all(d.n[a] > 1 for a in range(3))

@@ -86,7 +86,7 @@ def test_fused_direct_launch_matches_separate_kernels(tile, pbc, dt, steps):
     for _ in range(steps):      # drift away from the sorted order (no re-sort)
         r.step()
     J1, x1, u1, a1, nr = _fused(r)
-    J4, x4, u4, a4, nr4 = _fused(r, 4)          # slot-by-slot stores, 4 blocks per CU: same arithmetic
+    J4, x4, u4, a4, nr4 = _fused(r, 4)          # 4 blocks per CU (tile 4: two slots per thread): same arithmetic
     assert torch.equal(x1, x4) and torch.equal(u1, u4) and torch.equal(a1, a4) and nr4 == nr
     assert float((J4 - J1).norm() / J1.norm()) <= 1e-6
     J0, x0, u0, a0 = _separate(r)
