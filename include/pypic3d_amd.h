@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define PIC_ABI_VERSION 10
+#define PIC_ABI_VERSION 11
 #define PIC_MAX_SPECIES 16
 
 /* Scalar run description: StaticParameters + DynamicParameters fields the hot
@@ -242,11 +242,17 @@ int pic_tile_ranges(const PicDesc* desc, const PicSpecies* sp, int32_t tile, con
  * one atomic per distinct key and chunk), a scan of the counts, a pass that
  * writes perm from each chunk's reserved positions, and a gather of the
  * particles through perm.  Replaces pic_tile_keys + pic_sort_permutation +
- * pic_permute_particles + pic_tile_ranges.  workspace: the query below. */
+ * pic_permute_particles + pic_tile_ranges.  lookahead: the keys are taken at
+ * the predicted position x + lookahead * u (FP32; axes the species does not
+ * move stay put), 0 for the current position: a caller that re-sorts every k
+ * steps passes about k dt / 2, so the particles are closest to their sorted
+ * cells in the middle of the interval instead of at its start (any order
+ * gives the same results up to FP32 summation order).  workspace: the query
+ * below. */
 int64_t pic_tile_rebin_workspace_bytes(const PicDesc* desc, const PicSpecies* sp, int32_t tile);
 int pic_tile_rebin(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, int32_t tile,
-                   PicParticles* out, int32_t* starts, int32_t* perm, void* workspace, int64_t workspace_bytes,
-                   void* stream);
+                   double lookahead, PicParticles* out, int32_t* starts, int32_t* perm, void* workspace,
+                   int64_t workspace_bytes, void* stream);
 int64_t pic_sort_workspace_bytes(int64_t n);
 /* Stable radix sort of (key, slot) pairs: perm = stable argsort(keys);
  * keys_sorted receives the sorted keys. */

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libpypic3d_hip.so")
 MAX_SPECIES = 16
 PUSHERS = {"boris": 0, "higuera_cary": 1}
 FUSED_VARIANTS = 6
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 c_int = ctypes.c_int
 c_i32 = ctypes.c_int32
@@ -90,7 +90,7 @@ SIGNATURES = {
     "pic_tile_keys": (c_int, [DP, SP, PP, c_i32, c_vp, c_vp]),
     "pic_tile_ranges": (c_int, [DP, SP, c_i32, c_vp, c_vp, c_vp]),
     "pic_tile_rebin_workspace_bytes": (c_i64, [DP, SP, c_i32]),
-    "pic_tile_rebin": (c_int, [DP, SP, PP, c_i32, PP, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "pic_tile_rebin": (c_int, [DP, SP, PP, c_i32, c_dbl, PP, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "pic_sort_workspace_bytes": (c_i64, [c_i64]),
     "pic_sort_permutation": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i32, c_vp, c_i64, c_vp]),
     "pic_permute_particles": (c_int, [PP, PP, c_vp, c_i64, c_vp]),

@@ -113,7 +113,7 @@ __device__ inline int hash_insert(uint32_t* tkey, uint32_t key) {
     }
 }
 
-__global__ void rebin_count_kernel(PicDesc d, PicSpecies sp, PicParticles in, int T, TileKeyConst kc,
+__global__ void rebin_count_kernel(PicDesc d, PicSpecies sp, PicParticles in, int T, TileKeyConst kc, float ahead,
                                    uint32_t* __restrict__ count, uint32_t* __restrict__ keys, int64_t n) {
     __shared__ uint32_t tkey[kHashSize];
     __shared__ uint32_t tcnt[kHashSize];
@@ -128,8 +128,14 @@ __global__ void rebin_count_kernel(PicDesc d, PicSpecies sp, PicParticles in, in
         const bool in_range = i < n;
         uint32_t key = kEmpty;          // past the end: a run of its own, never inserted
         if (in_range) {
-            const float x[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
-            key = tile_key(d, T, kc, species_of(i, sp.slots, sp.count), in.active[i] != 0, x);
+            const int s = species_of(i, sp.slots, sp.count);
+            float x[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
+            if (ahead != 0.0f) {        // block-uniform
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    if (sp.update_x[s][a]) x[a] = fmaf(ahead, in.u[a][i], x[a]);
+            }
+            key = tile_key(d, T, kc, s, in.active[i] != 0, x);
             keys[i] = key;
         }
         int p, q;
@@ -433,8 +439,8 @@ int64_t pic_tile_rebin_workspace_bytes(const PicDesc* desc, const PicSpecies* sp
 }
 
 int pic_tile_rebin(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, int32_t tile,
-                   PicParticles* out, int32_t* starts, int32_t* perm, void* workspace, int64_t workspace_bytes,
-                   void* stream) {
+                   double lookahead, PicParticles* out, int32_t* starts, int32_t* perm, void* workspace,
+                   int64_t workspace_bytes, void* stream) {
     TileKeyConst kc;
     PIC_REQUIRE(tile_key_const(desc, tile, kc) && particles_ok(sp, in) && out && starts);
     const int64_t n = (int64_t)sp->count * sp->slots;
@@ -460,8 +466,9 @@ int pic_tile_rebin(const PicDesc* desc, const PicSpecies* sp, const PicParticles
     if (rc) return rc;
     const int64_t chunks = (n + kRebinChunk - 1) / kRebinChunk;
     if (n > 0) {
+        PIC_REQUIRE(std::isfinite(lookahead));
         hipLaunchKernelGGL(rebin_count_kernel, dim3((unsigned)chunks), dim3(kBlock), 0, s, *desc, *sp, *in, (int)tile,
-                           kc, pos, keys, n);
+                           kc, (float)lookahead, pos, keys, n);
         PIC_CHECK_LAUNCH();
     }
     hipLaunchKernelGGL(scan_sums_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, pos, bins, sums);

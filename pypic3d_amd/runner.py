@@ -85,6 +85,7 @@ class ElectrodynamicRunner:
         # launch shape of the fused direct kernel: 4 resident blocks per CU
         # (two slots per thread) at tile 4, one at tile 8
         self.direct_min_blocks = 1
+        self.sort_lookahead = None      # tile re-sort keys at x + t u; None: t = sort_every * dt / 2
         self.tile_ws, self.direct_fused = None, False
         if tile is not None:
             self.set_tile(tile)
@@ -244,8 +245,13 @@ class ElectrodynamicRunner:
         src, dst = self.cur, 1 - self.cur
         n = self.S * self.P
         if self.tile:
-            N.call("pic_tile_rebin", self.desc, self.sp, self._pp(src), self.tile, self._pp(dst), N.ptr(self.starts),
-                   N.ptr(self.perm), N.ptr(self.rebin_ws), self.rebin_ws.numel(), s)
+            # keys at the position half a re-sort interval ahead: the order is then
+            # freshest in the middle of the interval instead of at its start
+            ahead = self.sort_lookahead
+            if ahead is None:
+                ahead = 0.5 * self.sort_every * float(self.desc.dt)
+            N.call("pic_tile_rebin", self.desc, self.sp, self._pp(src), self.tile, ahead, self._pp(dst),
+                   N.ptr(self.starts), N.ptr(self.perm), N.ptr(self.rebin_ws), self.rebin_ws.numel(), s)
         else:
             ncell = self.desc.n[0] * self.desc.n[1] * self.desc.n[2]
             N.call("pic_cell_keys", self.desc, self.sp, self._pp(src), N.ptr(self.keys), s)
