@@ -394,7 +394,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--ppc", type=int, default=32)
-    ap.add_argument("--sort-every", type=int, default=6)
+    ap.add_argument("--sort-every", type=int, default=10)
     ap.add_argument("--tile", type=int, default=4, choices=(0, 4, 8),
                     help="cells per tile edge of the tile-ordered fused kernel (0: slot-ordered kernel)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
