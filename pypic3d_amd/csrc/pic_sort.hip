@@ -123,19 +123,36 @@ __global__ void rebin_count_kernel(PicDesc d, PicSpecies sp, PicParticles in, in
     }
     __syncthreads();
     const int64_t b0 = (int64_t)blockIdx.x * kRebinChunk;
+    // every item's loads first, so they share one memory round trip
+    float X[kRebinItems][3], U[kRebinItems][3];
+    bool A[kRebinItems];
+#pragma unroll
+    for (int k = 0; k < kRebinItems; ++k) {
+        const int64_t i = b0 + (int64_t)k * kBlock + threadIdx.x;
+        A[k] = false;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) X[k][a] = U[k][a] = 0.0f;
+        if (i < n) {
+            A[k] = in.active[i] != 0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) X[k][a] = in.x[a][i];
+            if (ahead != 0.0f) {        // uniform
+#pragma unroll
+                for (int a = 0; a < 3; ++a) U[k][a] = in.u[a][i];
+            }
+        }
+    }
+#pragma unroll
     for (int k = 0; k < kRebinItems; ++k) {
         const int64_t i = b0 + (int64_t)k * kBlock + threadIdx.x;
         const bool in_range = i < n;
         uint32_t key = kEmpty;          // past the end: a run of its own, never inserted
         if (in_range) {
             const int s = species_of(i, sp.slots, sp.count);
-            float x[3] = {in.x[0][i], in.x[1][i], in.x[2][i]};
-            if (ahead != 0.0f) {        // block-uniform
+            float x[3];
 #pragma unroll
-                for (int a = 0; a < 3; ++a)
-                    if (sp.update_x[s][a]) x[a] = fmaf(ahead, in.u[a][i], x[a]);
-            }
-            key = tile_key(d, T, kc, s, in.active[i] != 0, x);
+            for (int a = 0; a < 3; ++a) x[a] = sp.update_x[s][a] ? fmaf(ahead, U[k][a], X[k][a]) : X[k][a];
+            key = tile_key(d, T, kc, s, A[k], x);
             keys[i] = key;
         }
         int p, q;
