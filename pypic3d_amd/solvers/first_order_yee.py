@@ -74,6 +74,8 @@ def sigma_lines(desc, sigma3):
             lines = None
             break
         lines.append(line)
+    for k in [k for k, (refs, _, _) in _SIGMA_LINES.items() if any(r() is None for r in refs)]:
+        del _SIGMA_LINES[k]             # entries of profiles that no longer exist
     _SIGMA_LINES[key] = (tuple(weakref.ref(t) for t in sigma3), tuple(t._version for t in sigma3), lines)
     return lines
 
