@@ -14,7 +14,9 @@ update + boundaries), algorithmic bytes 50 per particle (read x,y,z,vx,vy,vz
 and the active byte, write them back) + 36 per cell (E, B gather reads and the
 J write), divided by the mean duration of the particle phase from HIP events
 recorded on the stream the kernel runs on (the phase is the fused kernel plus
-its rare-list kernel, about 0.15 ms).  traffic: calibrated FETCH_SIZE +
+its rare-list kernel, about 0.15 ms; the launch after each re-bin reads the
+particles through the re-bin's permutation, 4 more bytes per particle, and
+that permutation is never applied on its own).  traffic: calibrated FETCH_SIZE +
 WRITE_SIZE bytes per launch from the kernel's committed rocprofv3 summary in
 profiles/ for this workload (null when there is none).
 cpu_baseline: the FP64 NumPy oracle (oracle/) on a bounded sample of the same
@@ -483,6 +485,9 @@ def main():
     value = n_particles * args.steps / elapsed
     kern_ms = phases.get("particles")
     kern_bytes = 50 * n_particles + 36 * n_cells
+    if runner.tile and runner.defer_permute:
+        # the launch after each re-bin also reads the 4-byte source slot of every particle
+        kern_bytes += 4 * n_particles * sorts // args.steps
     achieved = kern_bytes / (kern_ms * 1e-3) / 1e9 if kern_ms else None
     step_bytes = 50 * n_particles + 156 * n_cells
     out = {

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define PIC_ABI_VERSION 11
+#define PIC_ABI_VERSION 12
 #define PIC_MAX_SPECIES 16
 
 /* Scalar run description: StaticParameters + DynamicParameters fields the hot
@@ -195,6 +195,19 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
                                    const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,
                                    const int32_t* starts, int32_t rounds, int32_t min_blocks, int32_t* workspace,
                                    void* stream);
+/* pic_fused_push_esirkepov_tiled on particles re-binned without their
+ * gather (pic_tile_rebin with out == NULL): slot i of the sorted order reads
+ * in's slot gather[i] (perm of that re-bin; starts index the sorted order)
+ * and writes out's slot i, so out holds the sorted particles after the push.
+ * Results equal pic_permute_particles(in, gather) followed by
+ * pic_fused_push_esirkepov_tiled: the particles bit for bit, J up to FP32
+ * summation order.  Exact grids only (pic_tile_exact_grid, desc->variant !=
+ * 1); -1 otherwise. */
+int pic_fused_push_esirkepov_tiled_gather(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                                          const int32_t* gather, const float* const E[3], const float* const B[3],
+                                          const float* const E_ext[3], const float* const B_ext[3],
+                                          PicParticles* out, float* const J[3], int32_t tile, const int32_t* starts,
+                                          int32_t rounds, int32_t min_blocks, int32_t* workspace, void* stream);
 /* Fused particle phase of time_loop_electrodynamic's direct branch
  * (evolve.py:44-67) for tile-sorted particles on an exact 3-D first-order grid
  * whose cell size equals the node spacing (desc->d == desc->sp_c): gather +
@@ -247,8 +260,9 @@ int pic_tile_ranges(const PicDesc* desc, const PicSpecies* sp, int32_t tile, con
  * move stay put), 0 for the current position: a caller that re-sorts every k
  * steps passes about k dt / 2, so the particles are closest to their sorted
  * cells in the middle of the interval instead of at its start (any order
- * gives the same results up to FP32 summation order).  workspace: the query
- * below. */
+ * gives the same results up to FP32 summation order).  out == NULL skips the
+ * gather: the caller applies perm itself (pic_permute_particles, or the next
+ * step's pic_fused_push_esirkepov_tiled_gather).  workspace: the query below. */
 int64_t pic_tile_rebin_workspace_bytes(const PicDesc* desc, const PicSpecies* sp, int32_t tile);
 int pic_tile_rebin(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, int32_t tile,
                    double lookahead, PicParticles* out, int32_t* starts, int32_t* perm, void* workspace,

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libpypic3d_hip.so")
 MAX_SPECIES = 16
 PUSHERS = {"boris": 0, "higuera_cary": 1}
 FUSED_VARIANTS = 6
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 c_int = ctypes.c_int
 c_i32 = ctypes.c_int32
@@ -82,6 +82,8 @@ SIGNATURES = {
     "pic_fused_push_esirkepov": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, PP, P3, c_vp]),
     "pic_fused_push_esirkepov_tiled": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, PP, P3, c_i32, c_vp, c_i32, c_i32,
                                                c_vp, c_vp]),
+    "pic_fused_push_esirkepov_tiled_gather": (c_int, [DP, SP, PP, c_vp, P3, P3, c_vp, c_vp, PP, P3, c_i32, c_vp,
+                                                      c_i32, c_i32, c_vp, c_vp]),
     "pic_fused_push_direct_tiled": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, PP, P3, c_i32, c_vp, c_i32, c_vp,
                                             c_vp]),
     "pic_tile_exact_grid": (c_int, [DP]),

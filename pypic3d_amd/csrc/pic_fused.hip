@@ -163,10 +163,10 @@ __device__ inline void periodic_wrap(double wind, double& x) {
 template <class Sink>
 __device__ inline void tile_particle_general(const PicDesc& d, const SpeciesK& k, const PicParticles& in,
                                              const FieldPtrs& fp, const PicParticles& out, const Sink& sink,
-                                             int64_t i) {
+                                             int64_t i, int64_t src) {
     double xo[3];
-    load_pos(in, i, xo);
-    float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
+    load_pos(in, src, xo);
+    float v[3] = {in.u[0][src], in.u[1][src], in.u[2][src]};
     CicAxis gc[3], gv[3];
     int anc[3], anv[3];
     float dc[3], dv[3];
@@ -318,10 +318,11 @@ __device__ inline void crossing_deposit_box(const SpeciesK& k, const CrossAxis& 
     crossing_component_box<2, kB>(k.pre[2], cz, cx, cy, p);
 }
 
-template <int kT, int kP, int kMinBlocks, int kG>
+template <int kT, int kP, int kMinBlocks, int kG, bool kGather>
 __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
     PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, PicParticles out, float* Jx, float* Jy, float* Jz,
-    const int32_t* __restrict__ starts, TileGeom tg, ExactGrid eg, int32_t* __restrict__ rare, int vec_ok, SpeciesTable st) {
+    const int32_t* __restrict__ starts, TileGeom tg, ExactGrid eg, int32_t* __restrict__ rare, int vec_ok, SpeciesTable st,
+    const int32_t* __restrict__ gather) {
     constexpr int kB = kT + 4;
     constexpr int kB3 = kB * kB * kB;
     constexpr int kFB = kT + 4;
@@ -381,7 +382,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
         // rewritten by the rare kernel.
         float PX[3][kP], PU[3][kP];
         uint32_t pact = 0u, oact = 0u;
-        if (full) {
+        if (!kGather && full) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 load_vec<kP>(in.x[a] + s0, PX[a]);
@@ -389,17 +390,33 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
             }
             if constexpr (kP == 4) pact = *reinterpret_cast<const uint32_t*>(in.active + s0);
             else pact = *reinterpret_cast<const uint16_t*>(in.active + s0);
+        } else if (kGather && full) {
+            // slot i's input is in.*[gather[i]]: the permutation of a re-bin,
+            // applied by this launch while it reads
+            float gf[kP];
+            load_vec<kP>(reinterpret_cast<const float*>(gather + s0), gf);
+#pragma unroll
+            for (int j = 0; j < kP; ++j) {
+                const int64_t src = __float_as_int(gf[j]);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    PX[a][j] = in.x[a][src];
+                    PU[a][j] = in.u[a][src];
+                }
+                pact |= (in.active[src] != 0 ? 1u : 0u) << (8 * j);
+            }
         } else {
 #pragma unroll
             for (int j = 0; j < kP; ++j) {
                 const int64_t i = s0 + j;
                 const bool inr = i >= begin && i < end;
+                const int64_t src = kGather ? (inr ? (int64_t)gather[i] : 0) : i;
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    PX[a][j] = inr ? in.x[a][i] : 0.0f;
-                    PU[a][j] = inr ? in.u[a][i] : 0.0f;
+                    PX[a][j] = inr ? in.x[a][src] : 0.0f;
+                    PU[a][j] = inr ? in.u[a][src] : 0.0f;
                 }
-                pact |= (inr && in.active[i] != 0 ? 1u : 0u) << (8 * j);
+                pact |= (inr && in.active[src] != 0 ? 1u : 0u) << (8 * j);
             }
         }
 #pragma unroll 1
@@ -622,13 +639,14 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
 __global__ __launch_bounds__(kThreads) void fused_tile_rare_kernel(PicDesc d, PicSpecies sp, PicParticles in,
                                                                    FieldPtrs fp, PicParticles out, float* Jx,
                                                                    float* Jy, float* Jz,
-                                                                   const int32_t* __restrict__ rare) {
+                                                                   const int32_t* __restrict__ rare,
+                                                                   const int32_t* __restrict__ gather) {
     const int n = rare[0];
     const GlobalSink sink{{Jx, Jy, Jz}};
     for (int q = blockIdx.x * kThreads + threadIdx.x; q < n; q += gridDim.x * kThreads) {
         const int64_t i = rare[1 + q];
         const SpeciesK k = species_constants(d, sp, species_of(i, sp.slots, sp.count));
-        tile_particle_general(d, k, in, fp, out, sink, i);
+        tile_particle_general(d, k, in, fp, out, sink, i, gather ? (int64_t)gather[i] : i);
     }
 }
 
@@ -1098,11 +1116,11 @@ int pic_tile_exact_grid(const PicDesc* desc) {
     return desc && desc_ok(desc) && make_exact_grid(*desc, eg) ? 1 : 0;
 }
 
-int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
-                                   const float* const E[3], const float* const B[3], const float* const E_ext[3],
-                                   const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,
-                                   const int32_t* starts, int32_t rounds, int32_t min_blocks, int32_t* workspace,
-                                   void* stream) {
+static int fused_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                                 const float* const E[3], const float* const B[3], const float* const E_ext[3],
+                                 const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,
+                                 const int32_t* starts, int32_t rounds, int32_t min_blocks, int32_t* workspace,
+                                 const int32_t* gather, void* stream) {
     PIC_REQUIRE(desc_ok(desc) && particles_ok(sp, in) && particles_ok(sp, out) && E && B && J && starts);
     PIC_REQUIRE(fused_kind(*desc) == 3 && (tile == 4 || tile == 8) &&
                 (rounds == 2 || rounds == 4 || rounds == 8));
@@ -1141,11 +1159,20 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
             vec_ok &= ((((uintptr_t)out->x[a]) | ((uintptr_t)out->u[a]) | ((uintptr_t)in->x[a]) |
                         ((uintptr_t)in->u[a])) & 15u) == 0;
         vec_ok &= ((((uintptr_t)out->active) | ((uintptr_t)in->active)) & 3u) == 0;
+        vec_ok &= (((uintptr_t)gather) & 15u) == 0;
         hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, workspace);
         const SpeciesTable stab = species_table(*desc, *sp);
 #define PIC_EXACT_LAUNCH(T, P, MB, G)                                                                               \
-        hipLaunchKernelGGL((fused_tile_exact_kernel<T, P, MB, G>), dim3((unsigned)blocks), dim3(kThreads), 0, st,   \
-                           *desc, *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace, vec_ok, stab)
+        do {                                                                                                        \
+            if (gather)                                                                                             \
+                hipLaunchKernelGGL((fused_tile_exact_kernel<T, P, MB, G, true>), dim3((unsigned)blocks),            \
+                                   dim3(kThreads), 0, st, *desc, *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg,  \
+                                   eg, workspace, vec_ok, stab, gather);                                            \
+            else                                                                                                    \
+                hipLaunchKernelGGL((fused_tile_exact_kernel<T, P, MB, G, false>), dim3((unsigned)blocks),           \
+                                   dim3(kThreads), 0, st, *desc, *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg,  \
+                                   eg, workspace, vec_ok, stab, gather);                                            \
+        } while (0)
 #define PIC_EXACT_MB(T, P, G)                                  \
         do {                                                   \
             if (min_blocks == 3) PIC_EXACT_LAUNCH(T, P, 3, G); \
@@ -1166,7 +1193,7 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
 #undef PIC_EXACT_LAUNCH
         const int64_t rare_grid = std::min<int64_t>(std::max<int64_t>(grid_1d(n, kThreads), 1), 1024);
         hipLaunchKernelGGL(fused_tile_rare_kernel, dim3((unsigned)rare_grid), dim3(kThreads), 0, st, *desc, *sp, *in,
-                           fp, *out, J[0], J[1], J[2], workspace);
+                           fp, *out, J[0], J[1], J[2], workspace, gather);
     } else if (tile == 8) {
         if (rounds == 2) PIC_TILED_MB(2, 8);
         else PIC_TILED_MB(4, 8);
@@ -1178,6 +1205,26 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
 #undef PIC_TILED_LAUNCH
     PIC_CHECK_LAUNCH();
     return 0;
+}
+
+int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                                   const float* const E[3], const float* const B[3], const float* const E_ext[3],
+                                   const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,
+                                   const int32_t* starts, int32_t rounds, int32_t min_blocks, int32_t* workspace,
+                                   void* stream) {
+    return fused_esirkepov_tiled(desc, sp, in, E, B, E_ext, B_ext, out, J, tile, starts, rounds, min_blocks,
+                                 workspace, nullptr, stream);
+}
+
+int pic_fused_push_esirkepov_tiled_gather(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                                          const int32_t* gather, const float* const E[3], const float* const B[3],
+                                          const float* const E_ext[3], const float* const B_ext[3],
+                                          PicParticles* out, float* const J[3], int32_t tile, const int32_t* starts,
+                                          int32_t rounds, int32_t min_blocks, int32_t* workspace, void* stream) {
+    ExactGrid eg;
+    PIC_REQUIRE(gather && desc && desc_ok(desc) && desc->variant != 1 && make_exact_grid(*desc, eg));
+    return fused_esirkepov_tiled(desc, sp, in, E, B, E_ext, B_ext, out, J, tile, starts, rounds, min_blocks,
+                                 workspace, gather, stream);
 }
 
 }  // extern "C"

@@ -459,7 +459,7 @@ int pic_tile_rebin(const PicDesc* desc, const PicSpecies* sp, const PicParticles
                    double lookahead, PicParticles* out, int32_t* starts, int32_t* perm, void* workspace,
                    int64_t workspace_bytes, void* stream) {
     TileKeyConst kc;
-    PIC_REQUIRE(tile_key_const(desc, tile, kc) && particles_ok(sp, in) && out && starts);
+    PIC_REQUIRE(tile_key_const(desc, tile, kc) && particles_ok(sp, in) && starts);
     const int64_t n = (int64_t)sp->count * sp->slots;
     const int64_t bins = rebin_bins(desc, sp, tile);
     PIC_REQUIRE(n < ((int64_t)1 << 31) && bins < ((int64_t)1 << 31));
@@ -468,8 +468,8 @@ int pic_tile_rebin(const PicDesc* desc, const PicSpecies* sp, const PicParticles
     const int64_t vol = (int64_t)tile * tile * tile;
     hipStream_t s = as_stream(stream);
     const int64_t nb = (int64_t)sp->count * (tiles + 1) + 1;
-    if (n > 0) {
-        PIC_REQUIRE(perm);
+    if (n > 0) PIC_REQUIRE(perm);
+    if (n > 0 && out) {
         for (int a = 0; a < 3; ++a)
             PIC_REQUIRE(out->x[a] && out->u[a] && out->x[a] != in->x[a] && out->u[a] != in->u[a]);
         PIC_REQUIRE(out->active && out->active != in->active);
@@ -496,6 +496,9 @@ int pic_tile_rebin(const PicDesc* desc, const PicSpecies* sp, const PicParticles
     PIC_CHECK_LAUNCH();
     if (n > 0) {
         hipLaunchKernelGGL(rebin_perm_kernel, dim3((unsigned)chunks), dim3(kBlock), 0, s, keys, pos, perm, n);
+        PIC_CHECK_LAUNCH();
+    }
+    if (n > 0 && out) {
         bool vec = (((uintptr_t)perm) & 15u) == 0 && (((uintptr_t)out->active) & 3u) == 0;
         for (int a = 0; a < 3; ++a) vec = vec && ((((uintptr_t)out->x[a]) | ((uintptr_t)out->u[a])) & 15u) == 0;
         if (vec)

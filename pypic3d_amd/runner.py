@@ -5,7 +5,11 @@
 preallocated ping-pong buffers: particle SoA planes alternate between two
 sets, E alternates between two buffers and B returns to its own buffer after
 the two half steps.  Optional cell re-sorting every ``sort_every`` steps keeps
-particles cache-local.  Per-phase HIP-event timing is available for the
+particles cache-local; in the tile-ordered Esirkepov branch on an exact grid
+the re-bin stops at its permutation and the next step's fused kernel reads
+the particles through it (the permutation is applied on its own only when
+the particles are read before that step, through ``cur`` or ``state()``).
+Per-phase HIP-event timing is available for the
 benchmark (events are recorded on the stream the kernels run on).
 """
 
@@ -53,7 +57,10 @@ class ElectrodynamicRunner:
             self.pos[0][a].copy_(xp[a])
             self.vel[0][a].copy_(up[a])
         self.act[0].copy_(act)
-        self.cur = 0
+        self._cur = 0
+        # a tile re-bin whose permutation the next fused step applies while it
+        # reads (self.perm; the particles still sit in the set self._cur)
+        self._pending = False
         self.E = [[as_field3(c, "E").clone() for c in E], [as_field3(c, "E").clone() for c in E]]
         self.B = [as_field3(c, "B").clone() for c in B]
         self.Bt = [b.clone() for b in self.B]
@@ -87,10 +94,26 @@ class ElectrodynamicRunner:
         self.direct_min_blocks = 1
         self.sort_lookahead = None      # tile re-sort keys at x + t u; None: t = sort_every * dt / 2
         self.tile_ws, self.direct_fused = None, False
+        self.defer_permute = False
         if tile is not None:
             self.set_tile(tile)
 
     # -------------------------------------------------------------- helpers
+    @property
+    def cur(self):
+        """Index of the particle set that holds the current particles (in
+        their sorted order: a deferred permutation is applied first)."""
+        self._materialize()
+        return self._cur
+
+    def _materialize(self):
+        if self._pending:
+            self._pending = False
+            src, dst = self._cur, 1 - self._cur
+            N.call("pic_permute_particles", self._pp(src), self._pp(dst), N.ptr(self.perm), self.S * self.P,
+                   N.stream_handle())
+            self._cur = dst
+
     def _pp(self, k):
         return pic_particles([self.pos[k][0], self.pos[k][1], self.pos[k][2]],
                              [self.vel[k][0], self.vel[k][1], self.vel[k][2]], self.act[k])
@@ -117,13 +140,21 @@ class ElectrodynamicRunner:
     # ----------------------------------------------------------------- step
     def step(self):
         d, s = self.desc, N.stream_handle()
-        src, dst = self.cur, 1 - self.cur
+        src, dst = self._cur, 1 - self._cur
+        if self._pending and not (self.esirkepov and self.tile):
+            self._materialize()
+            src, dst = self._cur, 1 - self._cur
         E, Et = self.E[self.ecur], self.E[1 - self.ecur]
         if self.esirkepov:
             for c in self.J:
                 N.call("pic_fill_f32", N.ptr(c), c.numel(), 0.0, s)
             ev = self._mark("particles")
-            if self.tile:
+            if self.tile and self._pending:
+                N.call("pic_fused_push_esirkepov_tiled_gather", d, self.sp, self._pp(src), N.ptr(self.perm), N.p3(E),
+                       N.p3(self.B), self.Ee, self.Be, self._pp(dst), N.p3(self.J), self.tile, N.ptr(self.starts),
+                       self.rounds, self.min_blocks, N.ptr(self.tile_ws), s)
+                self._pending = False
+            elif self.tile:
                 N.call("pic_fused_push_esirkepov_tiled", d, self.sp, self._pp(src), N.p3(E), N.p3(self.B), self.Ee,
                        self.Be, self._pp(dst), N.p3(self.J), self.tile, N.ptr(self.starts), self.rounds,
                        self.min_blocks, N.ptr(self.tile_ws), s)
@@ -169,7 +200,7 @@ class ElectrodynamicRunner:
         yee_b(d, Et, self.Bt, self.B, True, s, scratch=self.scratch, pml=self.pml_b)
         if ev:
             ev[1].record()
-        self.cur = dst
+        self._cur = dst
         self.ecur = 1 - self.ecur
         self.steps_done += 1
         if self.sort_every and self.steps_done % self.sort_every == 0:
@@ -213,8 +244,10 @@ class ElectrodynamicRunner:
     def set_tile(self, tile):
         """Switch to the tile-ordered kernel with `tile`^3-cell tiles (None:
         the slot-ordered kernel) and sort the particles accordingly."""
+        self._materialize()
         if tile is None:
             self.tile, self.starts, self.tile_ws, self.direct_fused = None, None, None, False
+            self.defer_permute = False
             return
         d = self.desc
         if self.esirkepov and not (d.shape_factor == 1 and all(d.n[a] > 1 for a in range(3))):
@@ -228,6 +261,9 @@ class ElectrodynamicRunner:
         self.direct_fused = (not self.esirkepov and d.shape_factor == 1 and all(d.n[a] > 1 for a in range(3))
                              and bool(N.lib().pic_tile_exact_grid(d))
                              and all(d.d[a] == d.sp_c[a] for a in range(3)))
+        # Esirkepov branch on an exact grid: a re-bin leaves its gather to the
+        # next step's fused kernel, which reads the particles through perm
+        self.defer_permute = (self.esirkepov and d.variant != 1 and bool(N.lib().pic_tile_exact_grid(d)))
         tiles = int(N.lib().pic_tile_count(d, self.tile))
         self.starts = torch.empty(self.S * (tiles + 1) + 1, dtype=torch.int32, device=self.device)
         # the tiled kernel's list of particles for its general route (1 + slots int32)
@@ -239,7 +275,8 @@ class ElectrodynamicRunner:
         tile mode a counting re-bin into tile-key order (the order inside one
         cell is unspecified; results do not depend on it), otherwise the
         stable sort into the reference's cell order.  perm[i] is the previous
-        slot of slot i."""
+        slot of slot i.  With defer_permute the re-bin stops at perm and the
+        next step (or the first access to ``cur``) applies it."""
         self._sort_buffers()
         s = N.stream_handle()
         src, dst = self.cur, 1 - self.cur
@@ -250,15 +287,21 @@ class ElectrodynamicRunner:
             ahead = self.sort_lookahead
             if ahead is None:
                 ahead = 0.5 * self.sort_every * float(self.desc.dt)
-            N.call("pic_tile_rebin", self.desc, self.sp, self._pp(src), self.tile, ahead, self._pp(dst),
-                   N.ptr(self.starts), N.ptr(self.perm), N.ptr(self.rebin_ws), self.rebin_ws.numel(), s)
+            defer = self.defer_permute
+            N.call("pic_tile_rebin", self.desc, self.sp, self._pp(src), self.tile, ahead,
+                   None if defer else self._pp(dst), N.ptr(self.starts), N.ptr(self.perm), N.ptr(self.rebin_ws),
+                   self.rebin_ws.numel(), s)
+            if defer:
+                self._pending = True
+                self.sorts_done += 1
+                return
         else:
             ncell = self.desc.n[0] * self.desc.n[1] * self.desc.n[2]
             N.call("pic_cell_keys", self.desc, self.sp, self._pp(src), N.ptr(self.keys), s)
             N.call("pic_sort_permutation", N.ptr(self.keys), N.ptr(self.keys_sorted), N.ptr(self.perm), n,
                    key_bits(self.S, ncell), N.ptr(self.sort_ws), self.sort_ws.numel(), s)
             N.call("pic_permute_particles", self._pp(src), self._pp(dst), N.ptr(self.perm), n, s)
-        self.cur = dst
+        self._cur = dst
         self.sorts_done += 1
 
     def state(self):

@@ -35,7 +35,9 @@ def _want_keys(x, act, p, tile):
     return np.where(act, tl * vol + loc, span) + np.arange(S)[:, None] * (span + 1), int(nt.prod()), span, vol
 
 
-def _rebin(desc, sp, parts, tile, ahead=0.0):
+def _rebin(desc, sp, parts, tile, ahead=0.0, gather=True):
+    """gather=False: the re-bin stops at starts and perm (out NULL); the
+    returned planes are then the input planes taken through perm here."""
     xp, up, ap = particle_planes(parts)
     S, P = xp[0].shape
     xs, xv = new_soa(S, P, "cuda")
@@ -45,10 +47,20 @@ def _rebin(desc, sp, parts, tile, ahead=0.0):
     starts = torch.full((S * (tiles + 1) + 1,), -1, dtype=torch.int32, device="cuda")
     perm = torch.full((S * P,), -1, dtype=torch.int32, device="cuda")
     ws = torch.empty(int(N.lib().pic_tile_rebin_workspace_bytes(desc, sp, tile)), dtype=torch.uint8, device="cuda")
-    N.call("pic_tile_rebin", desc, sp, pic_particles(xp, up, ap), tile, ahead,
-           pic_particles([xs[0], xs[1], xs[2]], [us[0], us[1], us[2]], ao), N.ptr(starts), N.ptr(perm), N.ptr(ws),
-           ws.numel(), N.stream_handle())
+    for t in (xs, us):
+        t.fill_(-1.0)
+    ao.fill_(True)
+    out = pic_particles([xs[0], xs[1], xs[2]], [us[0], us[1], us[2]], ao) if gather else None
+    N.call("pic_tile_rebin", desc, sp, pic_particles(xp, up, ap), tile, ahead, out, N.ptr(starts), N.ptr(perm),
+           N.ptr(ws), ws.numel(), N.stream_handle())
     torch.cuda.synchronize()
+    if not gather:
+        assert bool((xs == -1.0).all()) and bool((us == -1.0).all()) and bool(ao.all())    # left alone
+        p64 = perm.long()
+        for a in range(3):
+            xs[a].view(-1).copy_(xp[a].reshape(-1)[p64])
+            us[a].view(-1).copy_(up[a].reshape(-1)[p64])
+        ao.view(-1).copy_(ap.reshape(-1)[p64])
     return xs, us, ao, starts.cpu().numpy(), perm.cpu().numpy().astype(np.int64)
 
 
@@ -70,8 +82,8 @@ def _check(x, u, act, xs, us, ao, starts, perm, p, tile):
     np.testing.assert_array_equal(starts, want_starts)
 
 
-@pytest.mark.parametrize("tile", [4, 8])
-def test_rebin_of_unsorted_particles(tile):
+@pytest.mark.parametrize("tile,gather", [(4, True), (8, True), (4, False)])
+def test_rebin_of_unsorted_particles(tile, gather):
     static, dyn = workloads.make_parameters((18, 12, 9), wind=(9.0, 6.0, 4.5), dt=0.05)
     p = oracle_params(static, dyn)
     S, P = 2, 7000
@@ -82,7 +94,7 @@ def test_rebin_of_unsorted_particles(tile):
     cfg = SpeciesConfig(sp.charge, sp.mass, sp.weight, sp.update_x, sp.update_u)
     parts = tiled_particles_from_arrays(x, u, act)
     desc, spk = make_desc(static, dyn), make_species(cfg, P)
-    _check(x, u, act, *_rebin(desc, spk, parts, tile), p, tile)
+    _check(x, u, act, *_rebin(desc, spk, parts, tile, gather=gather), p, tile)
 
 
 def test_rebin_in_a_running_simulation():

@@ -71,6 +71,58 @@ def test_exact_tile_kernel_agrees_with_strided_tile_kernel(tile, rounds, min_blo
     assert float((J0 - J1).norm() / J1.norm()) <= 1e-6
 
 
+@pytest.mark.parametrize("tile,rounds,min_blocks,dt", [(4, 4, 4, None), (8, 2, 1, None), (4, 8, 4, 3.0)])
+def test_gathered_read_equals_permute_then_push(tile, rounds, min_blocks, dt):
+    """A re-bin that leaves its gather to the push (pic_tile_rebin without
+    out, then pic_fused_push_esirkepov_tiled_gather reading slot i from
+    perm[i]) against the permuted particles pushed by
+    pic_fused_push_esirkepov_tiled: identical particles and rare lists, J
+    equal up to FP32 summation order.  dt = 3 sends most particles through
+    the rare-list kernel, which reads through perm as well."""
+    specs = [workloads.SpeciesSpec(24, weight=0.01, vth=(0.2,) * 3),
+             workloads.SpeciesSpec(7, charge=1.0, mass=20.0, weight=0.03, vth=(0.05,) * 3)]
+    r, *_ = _runner((20, 16, 12), (20.0, 16.0, 12.0), specs, tile, rounds, min_blocks, dt=dt)
+    assert r.defer_permute
+    # drift, then a re-bin whose gather is deferred (one step at dt = 3, which
+    # is past the field solver's CFL limit)
+    for _ in range(4 if dt is None else 1):
+        r.step()
+    r.sort()
+    assert r._pending
+    src, dst = r._cur, 1 - r._cur
+    for c in r.J:
+        c.zero_()
+    r.tile_ws[0] = -7
+    N.call("pic_fused_push_esirkepov_tiled_gather", r.desc, r.sp, r._pp(src), N.ptr(r.perm), N.p3(r.E[r.ecur]),
+           N.p3(r.B), r.Ee, r.Be, r._pp(dst), N.p3(r.J), r.tile, N.ptr(r.starts), r.rounds, r.min_blocks,
+           N.ptr(r.tile_ws), N.stream_handle())
+    torch.cuda.synchronize()
+    Ja, xa, ua = torch.stack(r.J).clone(), r.pos[dst].clone(), r.vel[dst].clone()
+    aa, nra = r.act[dst].clone(), int(r.tile_ws[0])
+    Jb, xb, ub, ab, nrb = _launch(r, 0)             # r.cur applies perm (into dst), then the plain launch
+    assert not r._pending
+    assert nra == nrb and nra >= 0
+    if dt is not None:
+        assert nra > aa.numel() // 4
+    bits = lambda t: t.view(torch.int32)            # noqa: E731
+    assert torch.equal(bits(xa), bits(torch.stack(xb))) and torch.equal(bits(ua), bits(torch.stack(ub)))
+    assert torch.equal(aa, ab)
+    assert bool(torch.isfinite(Jb).all())
+    assert float((Ja - Jb).norm() / Jb.norm()) <= 1e-6
+
+
+def test_gathered_read_needs_an_exact_grid():
+    specs = [workloads.SpeciesSpec(8, weight=0.01, vth=(0.1,) * 3)]
+    r, *_ = _runner((12, 10, 8), (1.3, 1.1, 0.7), specs, 4, 4, 1, ext=False)
+    assert not r.defer_permute and not r._pending
+    src, dst = r.cur, 1 - r.cur
+    perm = torch.arange(r.S * r.P, dtype=torch.int32, device="cuda")
+    rc = N.lib().pic_fused_push_esirkepov_tiled_gather(
+        r.desc, r.sp, r._pp(src), N.ptr(perm), N.p3(r.E[r.ecur]), N.p3(r.B), r.Ee, r.Be, r._pp(dst), N.p3(r.J),
+        r.tile, N.ptr(r.starts), r.rounds, r.min_blocks, N.ptr(r.tile_ws), N.stream_handle())
+    assert rc == -1
+
+
 def test_non_exact_grid_uses_the_strided_kernel():
     specs = [workloads.SpeciesSpec(8, weight=0.01, vth=(0.1,) * 3)]
     r, *_ = _runner((12, 10, 8), (1.3, 1.1, 0.7), specs, 4, 4, 1, ext=False)
