@@ -227,6 +227,17 @@ int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const
                                 const float* const E[3], const float* const B[3], const float* const E_ext[3],
                                 const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,
                                 const int32_t* starts, int32_t min_blocks, int32_t* workspace, void* stream);
+/* pic_fused_push_direct_tiled reading slot i of the sorted order from in's
+ * slot gather[i] (the perm of a pic_tile_rebin called with out == NULL), as
+ * pic_fused_push_esirkepov_tiled_gather does for the Esirkepov branch:
+ * results equal pic_permute_particles(in, gather) followed by
+ * pic_fused_push_direct_tiled, the particles bit for bit, J up to FP32
+ * summation order. */
+int pic_fused_push_direct_tiled_gather(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                                       const int32_t* gather, const float* const E[3], const float* const B[3],
+                                       const float* const E_ext[3], const float* const B_ext[3], PicParticles* out,
+                                       float* const J[3], int32_t tile, const int32_t* starts, int32_t min_blocks,
+                                       int32_t* workspace, void* stream);
 
 /* ---- cell sort (particle_initialization.py:89-102, 140-160) ------------- */
 /* key = s*(N+1) + (ix*Ny+iy)*Nz+iz with ix = clip(floor((x+W/2)/dx),0,Nx-1);
@@ -262,7 +273,7 @@ int pic_tile_ranges(const PicDesc* desc, const PicSpecies* sp, int32_t tile, con
  * cells in the middle of the interval instead of at its start (any order
  * gives the same results up to FP32 summation order).  out == NULL skips the
  * gather: the caller applies perm itself (pic_permute_particles, or the next
- * step's pic_fused_push_esirkepov_tiled_gather).  workspace: the query below. */
+ * step's pic_fused_push_esirkepov_tiled_gather / pic_fused_push_direct_tiled_gather).  workspace: the query below. */
 int64_t pic_tile_rebin_workspace_bytes(const PicDesc* desc, const PicSpecies* sp, int32_t tile);
 int pic_tile_rebin(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in, int32_t tile,
                    double lookahead, PicParticles* out, int32_t* starts, int32_t* perm, void* workspace,

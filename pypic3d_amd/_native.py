@@ -84,6 +84,8 @@ SIGNATURES = {
                                                c_vp, c_vp]),
     "pic_fused_push_esirkepov_tiled_gather": (c_int, [DP, SP, PP, c_vp, P3, P3, c_vp, c_vp, PP, P3, c_i32, c_vp,
                                                       c_i32, c_i32, c_vp, c_vp]),
+    "pic_fused_push_direct_tiled_gather": (c_int, [DP, SP, PP, c_vp, P3, P3, c_vp, c_vp, PP, P3, c_i32, c_vp,
+                                                   c_i32, c_vp, c_vp]),
     "pic_fused_push_direct_tiled": (c_int, [DP, SP, PP, P3, P3, c_vp, c_vp, PP, P3, c_i32, c_vp, c_i32, c_vp,
                                             c_vp]),
     "pic_tile_exact_grid": (c_int, [DP]),

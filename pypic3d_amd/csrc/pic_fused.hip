@@ -392,9 +392,16 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
             else pact = *reinterpret_cast<const uint16_t*>(in.active + s0);
         } else if (kGather && full) {
             // slot i's input is in.*[gather[i]]: the permutation of a re-bin,
-            // applied by this launch while it reads
+            // applied by this launch while it reads.  The flags are loaded
+            // first and combined only after every load has been issued (the
+            // scheduling barrier keeps the combination from being hoisted
+            // between the loads, where each use of a flag would wait for all
+            // loads issued before it)
             float gf[kP];
             load_vec<kP>(reinterpret_cast<const float*>(gather + s0), gf);
+            uint32_t ab[kP];
+#pragma unroll
+            for (int j = 0; j < kP; ++j) ab[j] = in.active[__float_as_int(gf[j])];
 #pragma unroll
             for (int j = 0; j < kP; ++j) {
                 const int64_t src = __float_as_int(gf[j]);
@@ -403,8 +410,10 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
                     PX[a][j] = in.x[a][src];
                     PU[a][j] = in.u[a][src];
                 }
-                pact |= (in.active[src] != 0 ? 1u : 0u) << (8 * j);
             }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < kP; ++j) pact |= (ab[j] != 0u ? 1u : 0u) << (8 * j);
         } else {
 #pragma unroll
             for (int j = 0; j < kP; ++j) {
@@ -664,10 +673,11 @@ __global__ void zero_count_kernel(int32_t* c) { *c = 0; }
 // a thread's four slots are stored with one 16-byte store per plane at the
 // end (24 more registers); otherwise each slot is stored as it finishes,
 // which keeps the kernel at 128 VGPRs for 4 waves per SIMD.
-template <int kT, int kMinBlocks, bool kVec, int kP = 4>
+template <int kT, int kMinBlocks, bool kVec, int kP = 4, bool kGather = false>
 __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel(
     PicDesc d, PicSpecies sp, PicParticles in, FieldPtrs fp, PicParticles out, float* Jx, float* Jy, float* Jz,
-    const int32_t* __restrict__ starts, TileGeom tg, ExactGrid eg, int32_t* __restrict__ rare, int vec_ok, SpeciesTable st) {
+    const int32_t* __restrict__ starts, TileGeom tg, ExactGrid eg, int32_t* __restrict__ rare, int vec_ok, SpeciesTable st,
+    const int32_t* __restrict__ gather) {
     constexpr int kB = kT + 5;
     constexpr int kB3 = kB * kB * kB;
     constexpr int kFB = kT + 4;
@@ -724,7 +734,7 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
         uint32_t pact = 0u, oact = 0u;
         const bool full = vec_ok && s0 >= begin && s0 + kP <= end;
         if constexpr (kVec) {
-            if (full) {
+            if (!kGather && full) {
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     load_vec<kP>(in.x[a] + s0, OX[a]);
@@ -732,17 +742,32 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
                 }
                 if constexpr (kP == 4) pact = *reinterpret_cast<const uint32_t*>(in.active + s0);
                 else pact = *reinterpret_cast<const uint16_t*>(in.active + s0);
+            } else if (kGather && full) {
+                // slot i's input is in.*[gather[i]] (a re-bin's permutation)
+                float gf[kP];
+                load_vec<kP>(reinterpret_cast<const float*>(gather + s0), gf);
+#pragma unroll
+                for (int j = 0; j < kP; ++j) {
+                    const int64_t src = __float_as_int(gf[j]);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        OX[a][j] = in.x[a][src];
+                        OU[a][j] = in.u[a][src];
+                    }
+                    pact |= (in.active[src] != 0 ? 1u : 0u) << (8 * j);
+                }
             } else {
 #pragma unroll
                 for (int j = 0; j < kP; ++j) {
                     const int64_t i = s0 + j;
                     const bool inr = i >= begin && i < end;
+                    const int64_t src = kGather ? (inr ? (int64_t)gather[i] : 0) : i;
 #pragma unroll
                     for (int a = 0; a < 3; ++a) {
-                        OX[a][j] = inr ? in.x[a][i] : 0.0f;
-                        OU[a][j] = inr ? in.u[a][i] : 0.0f;
+                        OX[a][j] = inr ? in.x[a][src] : 0.0f;
+                        OU[a][j] = inr ? in.u[a][src] : 0.0f;
                     }
-                    pact |= (inr && in.active[i] != 0 ? 1u : 0u) << (8 * j);
+                    pact |= (inr && in.active[src] != 0 ? 1u : 0u) << (8 * j);
                 }
             }
         }
@@ -770,11 +795,12 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
             do {
             if constexpr (!kVec) {
                 if (i < begin || i >= end) break;
-                act0 = in.active[i] != 0;
+                const int64_t src = kGather ? (int64_t)gather[i] : i;
+                act0 = in.active[src] != 0;
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    xf[a] = in.x[a][i];
-                    v[a] = in.u[a][i];
+                    xf[a] = in.x[a][src];
+                    v[a] = in.u[a][src];
                 }
                 loads_together(xf, v);
             }
@@ -951,15 +977,17 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_direct_kernel
 __global__ __launch_bounds__(kThreads) void fused_direct_rare_kernel(PicDesc d, PicSpecies sp, PicParticles in,
                                                                      FieldPtrs fp, PicParticles out, float* Jx,
                                                                      float* Jy, float* Jz,
-                                                                     const int32_t* __restrict__ rare) {
+                                                                     const int32_t* __restrict__ rare,
+                                                                     const int32_t* __restrict__ gather) {
     const int n = rare[0];
     const GlobalSink sink{{Jx, Jy, Jz}};
     for (int q = blockIdx.x * kThreads + threadIdx.x; q < n; q += gridDim.x * kThreads) {
         const int64_t i = rare[1 + q];
+        const int64_t src = gather ? (int64_t)gather[i] : i;      // input slot (a re-bin's permutation)
         const SpeciesK k = species_constants(d, sp, species_of(i, sp.slots, sp.count));
         double pos[3];
-        load_pos(in, i, pos);
-        float v[3] = {in.u[0][i], in.u[1][i], in.u[2][i]};
+        load_pos(in, src, pos);
+        float v[3] = {in.u[0][src], in.u[1][src], in.u[2][src]};
         float E[3], B[3];
         CicAxis gc[3], gv[3];
         for (int a = 0; a < 3; ++a) {
@@ -1058,10 +1086,11 @@ int pic_fused_push_esirkepov(const PicDesc* desc, const PicSpecies* sp, const Pi
     return 0;
 }
 
-int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
-                                const float* const E[3], const float* const B[3], const float* const E_ext[3],
-                                const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,
-                                const int32_t* starts, int32_t min_blocks, int32_t* workspace, void* stream) {
+static int fused_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                              const float* const E[3], const float* const B[3], const float* const E_ext[3],
+                              const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,
+                              const int32_t* starts, int32_t min_blocks, int32_t* workspace, const int32_t* gather,
+                              void* stream) {
     PIC_REQUIRE(desc_ok(desc) && particles_ok(sp, in) && particles_ok(sp, out) && E && B && J && starts && workspace);
     PIC_REQUIRE(fused_kind(*desc) == 3 && (tile == 4 || tile == 8) && (min_blocks == 1 || min_blocks == 4));
     ExactGrid eg;
@@ -1088,14 +1117,21 @@ int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const
         vec_ok &= ((((uintptr_t)out->x[a]) | ((uintptr_t)out->u[a]) | ((uintptr_t)in->x[a]) |
                     ((uintptr_t)in->u[a])) & 15u) == 0;
     vec_ok &= ((((uintptr_t)out->active) | ((uintptr_t)in->active)) & 3u) == 0;
+    vec_ok &= (((uintptr_t)gather) & 15u) == 0;
     hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, st, workspace);
     const SpeciesTable stab = species_table(*desc, *sp);
-#define PIC_DIRECT_LAUNCH2(T, MB, V, P)                                                                           \
-    hipLaunchKernelGGL((fused_tile_direct_kernel<T, MB, V, P>), dim3((unsigned)blocks), dim3(kThreads), 0, st, *desc, \
-                       *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace, vec_ok, stab)
-#define PIC_DIRECT_LAUNCH(T, MB, V)                                                                               \
-    hipLaunchKernelGGL((fused_tile_direct_kernel<T, MB, V>), dim3((unsigned)blocks), dim3(kThreads), 0, st, *desc,   \
-                       *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace, vec_ok, stab)
+#define PIC_DIRECT_LAUNCH2(T, MB, V, P)                                                                              \
+    do {                                                                                                             \
+        if (gather)                                                                                                  \
+            hipLaunchKernelGGL((fused_tile_direct_kernel<T, MB, V, P, true>), dim3((unsigned)blocks), dim3(kThreads), \
+                               0, st, *desc, *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg, workspace, vec_ok, \
+                               stab, gather);                                                                        \
+        else                                                                                                         \
+            hipLaunchKernelGGL((fused_tile_direct_kernel<T, MB, V, P, false>), dim3((unsigned)blocks),               \
+                               dim3(kThreads), 0, st, *desc, *sp, *in, fp, *out, J[0], J[1], J[2], starts, tg, eg,   \
+                               workspace, vec_ok, stab, gather);                                                     \
+    } while (0)
+#define PIC_DIRECT_LAUNCH(T, MB, V) PIC_DIRECT_LAUNCH2(T, MB, V, 4)
     if (tile == 8) {
         if (min_blocks == 4) PIC_DIRECT_LAUNCH(8, 4, false);
         else PIC_DIRECT_LAUNCH(8, 1, true);
@@ -1104,11 +1140,30 @@ int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const
         else PIC_DIRECT_LAUNCH(4, 1, true);
     }
 #undef PIC_DIRECT_LAUNCH
+#undef PIC_DIRECT_LAUNCH2
     const int64_t rare_grid = std::min<int64_t>(std::max<int64_t>(grid_1d(n, kThreads), 1), 1024);
     hipLaunchKernelGGL(fused_direct_rare_kernel, dim3((unsigned)rare_grid), dim3(kThreads), 0, st, *desc, *sp, *in,
-                       fp, *out, J[0], J[1], J[2], workspace);
+                       fp, *out, J[0], J[1], J[2], workspace, gather);
     PIC_CHECK_LAUNCH();
     return 0;
+}
+
+int pic_fused_push_direct_tiled(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                                const float* const E[3], const float* const B[3], const float* const E_ext[3],
+                                const float* const B_ext[3], PicParticles* out, float* const J[3], int32_t tile,
+                                const int32_t* starts, int32_t min_blocks, int32_t* workspace, void* stream) {
+    return fused_direct_tiled(desc, sp, in, E, B, E_ext, B_ext, out, J, tile, starts, min_blocks, workspace, nullptr,
+                              stream);
+}
+
+int pic_fused_push_direct_tiled_gather(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
+                                       const int32_t* gather, const float* const E[3], const float* const B[3],
+                                       const float* const E_ext[3], const float* const B_ext[3], PicParticles* out,
+                                       float* const J[3], int32_t tile, const int32_t* starts, int32_t min_blocks,
+                                       int32_t* workspace, void* stream) {
+    PIC_REQUIRE(gather);
+    return fused_direct_tiled(desc, sp, in, E, B, E_ext, B_ext, out, J, tile, starts, min_blocks, workspace, gather,
+                              stream);
 }
 
 int pic_tile_exact_grid(const PicDesc* desc) {

@@ -5,8 +5,8 @@
 preallocated ping-pong buffers: particle SoA planes alternate between two
 sets, E alternates between two buffers and B returns to its own buffer after
 the two half steps.  Optional cell re-sorting every ``sort_every`` steps keeps
-particles cache-local; in the tile-ordered Esirkepov branch on an exact grid
-the re-bin stops at its permutation and the next step's fused kernel reads
+particles cache-local; with the tile-ordered fused kernels (exact grids) the
+re-bin stops at its permutation and the next step's fused kernel reads
 the particles through it (the permutation is applied on its own only when
 the particles are read before that step, through ``cur`` or ``state()``).
 Per-phase HIP-event timing is available for the
@@ -141,7 +141,7 @@ class ElectrodynamicRunner:
     def step(self):
         d, s = self.desc, N.stream_handle()
         src, dst = self._cur, 1 - self._cur
-        if self._pending and not (self.esirkepov and self.tile):
+        if self._pending and not (self.tile and (self.esirkepov or self.direct_fused)):
             self._materialize()
             src, dst = self._cur, 1 - self._cur
         E, Et = self.E[self.ecur], self.E[1 - self.ecur]
@@ -169,9 +169,15 @@ class ElectrodynamicRunner:
             for c in self.J:
                 N.call("pic_fill_f32", N.ptr(c), c.numel(), 0.0, s)
             ev = self._mark("particles")
-            N.call("pic_fused_push_direct_tiled", d, self.sp, self._pp(src), N.p3(E), N.p3(self.B), self.Ee, self.Be,
-                   self._pp(dst), N.p3(self.J), self.tile, N.ptr(self.starts), self.direct_min_blocks,
-                   N.ptr(self.tile_ws), s)
+            if self._pending:
+                N.call("pic_fused_push_direct_tiled_gather", d, self.sp, self._pp(src), N.ptr(self.perm), N.p3(E),
+                       N.p3(self.B), self.Ee, self.Be, self._pp(dst), N.p3(self.J), self.tile, N.ptr(self.starts),
+                       self.direct_min_blocks, N.ptr(self.tile_ws), s)
+                self._pending = False
+            else:
+                N.call("pic_fused_push_direct_tiled", d, self.sp, self._pp(src), N.p3(E), N.p3(self.B), self.Ee,
+                       self.Be, self._pp(dst), N.p3(self.J), self.tile, N.ptr(self.starts), self.direct_min_blocks,
+                       N.ptr(self.tile_ws), s)
             if ev:
                 ev[1].record()
             ev = self._mark("current_fold")
@@ -261,9 +267,10 @@ class ElectrodynamicRunner:
         self.direct_fused = (not self.esirkepov and d.shape_factor == 1 and all(d.n[a] > 1 for a in range(3))
                              and bool(N.lib().pic_tile_exact_grid(d))
                              and all(d.d[a] == d.sp_c[a] for a in range(3)))
-        # Esirkepov branch on an exact grid: a re-bin leaves its gather to the
-        # next step's fused kernel, which reads the particles through perm
-        self.defer_permute = (self.esirkepov and d.variant != 1 and bool(N.lib().pic_tile_exact_grid(d)))
+        # a re-bin leaves its gather to the next step's fused kernel, which reads
+        # the particles through perm (both branches' fused kernels on exact grids)
+        self.defer_permute = self.direct_fused or (self.esirkepov and d.variant != 1
+                                                   and bool(N.lib().pic_tile_exact_grid(d)))
         tiles = int(N.lib().pic_tile_count(d, self.tile))
         self.starts = torch.empty(self.S * (tiles + 1) + 1, dtype=torch.int32, device=self.device)
         # the tiled kernel's list of particles for its general route (1 + slots int32)

@@ -103,6 +103,44 @@ def test_fused_direct_launch_matches_separate_kernels(tile, pbc, dt, steps):
     assert float((J0 - J1).norm() / J0.norm()) <= 1e-6
 
 
+@pytest.mark.parametrize("tile,min_blocks,pbc,dt", [(4, 4, (0, 0, 0), None), (4, 1, (1, 2, 0), None),
+                                                    (8, 1, (0, 0, 0), None), (8, 4, (0, 0, 0), None),
+                                                    (4, 4, (0, 0, 0), 3.0)])
+def test_gathered_direct_launch_equals_permute_then_launch(tile, min_blocks, pbc, dt):
+    """A re-bin that leaves its gather to the fused direct launch
+    (pic_fused_push_direct_tiled_gather reads slot i from perm[i]) against
+    the permuted particles through pic_fused_push_direct_tiled: identical
+    particles and rare lists, J equal up to FP32 summation order; the four
+    launch shapes (16-byte group loads, 8-byte groups, slot-by-slot loads)
+    and the rare-list route (dt = 3: 5 % of the particles) all read through
+    perm."""
+    r = _runner((20, 16, 12), (20.0, 16.0, 12.0), pbc, tile, dt=dt)
+    assert r.direct_fused and r.defer_permute
+    for _ in range(3 if dt is None else 1):
+        r.step()
+    r.sort()
+    assert r._pending
+    src, dst = r._cur, 1 - r._cur
+    for c in r.J:
+        c.zero_()
+    r.tile_ws[0] = -7
+    N.call("pic_fused_push_direct_tiled_gather", r.desc, r.sp, r._pp(src), N.ptr(r.perm), N.p3(r.E[r.ecur]),
+           N.p3(r.B), r.Ee, r.Be, r._pp(dst), N.p3(r.J), r.tile, N.ptr(r.starts), min_blocks, N.ptr(r.tile_ws),
+           N.stream_handle())
+    torch.cuda.synchronize()
+    Ja, xa, ua, aa, nra = (torch.stack(r.J).clone(), r.pos[dst].clone(), r.vel[dst].clone(), r.act[dst].clone(),
+                           int(r.tile_ws[0]))
+    Jb, xb, ub, ab, nrb = _fused(r, min_blocks)      # r.cur applies perm (into dst), then the plain launch
+    assert not r._pending
+    assert nra == nrb and nra > 0
+    if dt is not None:
+        assert nra > aa.numel() // 50
+    bits = lambda t: t.view(torch.int32)            # noqa: E731
+    assert torch.equal(bits(xa), bits(xb)) and torch.equal(bits(ua), bits(ub)) and torch.equal(aa, ab)
+    assert bool(torch.isfinite(Jb).all())
+    assert float((Ja - Jb).norm() / Jb.norm()) <= 1e-6
+
+
 def test_fused_direct_with_frozen_ions_and_drifting_beam():
     """Three species: a drifting beam, a thermal species and frozen ions
     (update_x = update_u = False, no push and no motion, but a deposit)."""
