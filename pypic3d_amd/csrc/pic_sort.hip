@@ -98,10 +98,15 @@ __global__ void tile_keys_kernel(PicDesc d, PicSpecies sp, PicParticles in, int 
 
 constexpr int kRebinItems = 4;
 constexpr int kRebinChunk = kBlock * kRebinItems;
-constexpr int kHashSize = 2 * kRebinChunk;          // load factor <= 1/2
+// one slot per item: a chunk of nearly sorted particles has a few dozen keys, and
+// the table cannot overflow (a random order fills it, at the cost of long probes)
+constexpr int kHashSize = kRebinChunk;
+static_assert((kHashSize & (kHashSize - 1)) == 0, "the hash table size is a power of two");
 constexpr uint32_t kEmpty = 0xffffffffu;
 
-__device__ inline int hash_slot(uint32_t key) { return (int)((key * 2654435761u) >> 21) & (kHashSize - 1); }
+__device__ inline int hash_slot(uint32_t key) {
+    return (int)((key * 2654435761u) >> (32 - __builtin_ctz(kHashSize))) & (kHashSize - 1);
+}
 
 // slot of key in the table (inserting it), by linear probing
 __device__ inline int hash_insert(uint32_t* tkey, uint32_t key) {
@@ -197,9 +202,25 @@ __global__ void rebin_perm_kernel(const uint32_t* __restrict__ keys, uint32_t* _
         slot[k] = in_range ? sl : -1;
         rank[k] = base + (uint32_t)(p - 1);
     }
+    // the chunk's base for each of its keys: the occupied table slots are listed
+    // first, so the returning global atomics go out in one round
+    __shared__ int nlist;
+    __shared__ uint16_t list[kHashSize];
+    if (threadIdx.x == 0) nlist = 0;
     __syncthreads();
-    for (int e = threadIdx.x; e < kHashSize; e += kBlock)
-        if (tcnt[e]) tcnt[e] = atomicAdd(&pos[tkey[e]], tcnt[e]);     // the chunk's base for the key
+    for (int e = threadIdx.x; e < kHashSize; e += kBlock) {
+        const bool used = tcnt[e] != 0u;
+        const unsigned long long m = __ballot(used);
+        int first = 0;
+        if (lane == 0 && m) first = atomicAdd(&nlist, __popcll(m));
+        first = __shfl(first, 0);
+        if (used) list[first + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)e;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nlist; q += kBlock) {
+        const int e = list[q];
+        tcnt[e] = atomicAdd(&pos[tkey[e]], tcnt[e]);
+    }
     __syncthreads();
     for (int k = 0; k < kRebinItems; ++k) {
         if (slot[k] < 0) continue;
