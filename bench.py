@@ -5,8 +5,8 @@ vth = 0.05c, 32 particles per cell (536,870,912 particles, SoA FP32), first
 order (CIC) Esirkepov current, relativistic Boris, Yee B/2 -> E -> B/2,
 omega_p dt = 0.1, cfl 0.5.  A step is one time_loop_electrodynamic call
 (evolve.py:16-103) executed by the allocation-free runner on tile-sorted
-particles (4^3-cell tiles by default), with a stable tile re-sort every
---sort-every steps (counted in the timed region).
+particles (4^3-cell tiles by default), with a tile re-bin every
+--sort-every steps (default 6; counted in the timed region).
 
 Prints ONE JSON line.  value = particle-steps/s over the timed steps.
 roofline: the fused particle kernel (gather + Boris + Esirkepov + position
@@ -178,7 +178,7 @@ def extras(n, ppc, steps, seed):
         static, dyn, species = workloads.config_warm_3d(n=n, ppc=ppc, vth=0.05, current_deposition="j_from_rhov")
         static = static._replace(current_filter=filt)
         parts, cfg = workloads.plasma(static, dyn, species, seed=seed)
-        r = ElectrodynamicRunner(parts, cfg, workloads.empty_fields(static, dyn), static, dyn, sort_every=6, tile=4)
+        r = ElectrodynamicRunner(parts, cfg, workloads.empty_fields(static, dyn), static, dyn, sort_every=4, tile=4)
         del parts
         ms = _timed(r.step, steps)
         npart = r.S * r.P
@@ -396,7 +396,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--ppc", type=int, default=32)
-    ap.add_argument("--sort-every", type=int, default=10)
+    ap.add_argument("--sort-every", type=int, default=6)
     ap.add_argument("--tile", type=int, default=4, choices=(0, 4, 8),
                     help="cells per tile edge of the tile-ordered fused kernel (0: slot-ordered kernel)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
