@@ -21,6 +21,10 @@ struct ExactGrid {
     float inv_d[3];
     float xlo[3], xhi[3];
     double tol[3];      // band around a face inside which a new FP64 position takes the queue
+    // FP32 form of the same test: an FP32 offset (x - node) + v dt in
+    // [band_lo, band_hi) puts the FP64 new position in [node + tol, node + sp - tol)
+    // (the FP32 offset is within 2^-23 sp of the FP64 one; the band is 2 tol wide)
+    float band_lo[3], band_hi[3];
 };
 
 // Center-line anchor and offset of x (xlo <= x < xhi): the FP32 estimate is
@@ -74,6 +78,11 @@ bool make_exact_grid(const PicDesc& d, ExactGrid& eg) {
         // floor((x - org)/sp) in FP64 can differ from the exact bracket only within a
         // few FP64 ulps of the quotient (|quotient| < 2^22 here): a band of 2^-20 sp is ample
         eg.tol[a] = std::ldexp(sp, -20);
+        eg.band_lo[a] = (float)std::ldexp(sp, -19);                      // exact
+        const double hd = sp - std::ldexp(sp, -19);
+        float hf = (float)hd;
+        if ((double)hf > hd) hf = std::nextafter(hf, 0.0f);                // rounded down
+        eg.band_hi[a] = hf;
     }
     return true;
 }

@@ -489,7 +489,10 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
                 if ((k.uu >> a) & 1u) v[a] = nv[a];
             // ---- new position (Esirkepov.py:125-127, FP64) and regular-case test: the
             // new position stays in the old cell, away from its faces by more than the
-            // FP64 rounding of floor((x - origin)/spacing)
+            // FP64 rounding of floor((x - origin)/spacing).  The test runs on the FP32
+            // offset (x - node) + v dt with a band twice as wide (ExactGrid::band_*),
+            // which implies the FP64 one; a particle inside the extra band width takes
+            // the crossing queue, which treats a stay in the cell like any other step
             double xn[3];
             bool regular = true, short_step = true;
             float S0[3][2], D[3];
@@ -497,9 +500,9 @@ __global__ __launch_bounds__(kThreads, kMinBlocks) void fused_tile_exact_kernel(
             for (int a = 0; a < 3; ++a) {
                 const bool ux = (k.ux >> a) & 1u;
                 xn[a] = ux ? step_pos((double)xf[a], (double)v[a], d.dt) : (double)xf[a];
-                const double lo = (double)nodec[a] + eg.tol[a], hi = (double)(nodec[a] + eg.sp_c[a]) - eg.tol[a];
-                regular = regular && (!ux || (xn[a] >= lo && xn[a] < hi)) && anc[a] >= 0 && anc[a] + 1 < d.dims[a];
                 const float dvt = ux ? v[a] * fdt : 0.0f;
+                const float e = dc[a] + dvt;      // a frozen axis tests its offset the same way
+                regular = regular && e >= eg.band_lo[a] && e < eg.band_hi[a] && anc[a] >= 0 && anc[a] + 1 < d.dims[a];
                 short_step = short_step && fabsf(dvt) < 0.99f * eg.sp_c[a];   // anchor moves by one node at most
                 const float r0 = dc[a] * eg.inv_d[a];
                 S0[a][0] = 1.0f - r0;
