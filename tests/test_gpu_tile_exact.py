@@ -123,6 +123,45 @@ def test_gathered_read_needs_an_exact_grid():
     assert rc == -1
 
 
+def test_new_positions_next_to_a_face_take_the_crossing_route():
+    """The fast loop tests "stays in its cell" on the FP32 offset
+    (x - node) + v dt against a band of 2^-19 spacings at each face, twice the
+    FP64 test's 2^-20: a particle whose FP64 new position lies between the
+    two bands stays in its cell but goes through the crossing queue.  Frozen
+    velocities (update_u = False) put new positions 1.5 * 2^-20 inside a face
+    on purpose; the kernel's particles equal the strided kernel's bit for bit
+    and J agrees to FP32 summation order."""
+    specs = [workloads.SpeciesSpec(16, weight=0.01, vth=(0.1,) * 3, update_u=False)]
+    r, *_ = _runner((20, 16, 12), (20.0, 16.0, 12.0), specs, 4, 4, 4, ext=False)
+    k = r.cur
+    dt, tol = float(r.desc.dt), 2.0 ** -20
+    act = r.act[k].view(-1)
+    idx = torch.nonzero(act).view(-1)[::37][:200]
+    for a in range(3):
+        org, sp = float(r.desc.org_c[a]), float(r.desc.sp_c[a])
+        assert sp == 1.0
+        x = r.pos[k][a].view(-1)
+        node = torch.floor((x[idx].double() - org) / sp) * sp + org
+        low = (torch.arange(idx.numel(), device="cuda") + a) % 2 == 0
+        x0 = node + torch.where(low, 0.25, 0.75)                  # exact in FP32, same cell
+        target = torch.where(low, node + 1.5 * tol, node + sp - 1.5 * tol)
+        x[idx] = x0.float()
+        r.vel[k][a].view(-1)[idx] = ((target - x0) / dt).float()
+    J1, x1, u1, a1, _ = _launch(r, 1)
+    J0, x0_, u0, a0, nr0 = _launch(r, 0)
+    # the planted particles stayed in their cells, inside the band between the tests
+    for a in range(3):
+        org = float(r.desc.org_c[a])
+        xn = x0_[a].view(-1)[idx].double()
+        off = xn - (torch.floor(xn - org) + org)
+        assert bool(((off > 0.5 * tol) & (off < 3 * tol) | (off > 1 - 3 * tol) & (off < 1 - 0.5 * tol)).all())
+    for p, q in zip(x0_ + u0, x1 + u1):
+        assert torch.equal(p, q)
+    assert torch.equal(a0, a1)
+    assert 0 <= nr0 < a0.numel() // 10
+    assert float((J0 - J1).norm() / J1.norm()) <= 1e-6
+
+
 def test_non_exact_grid_uses_the_strided_kernel():
     specs = [workloads.SpeciesSpec(8, weight=0.01, vth=(0.1,) * 3)]
     r, *_ = _runner((12, 10, 8), (1.3, 1.1, 0.7), specs, 4, 4, 1, ext=False)
