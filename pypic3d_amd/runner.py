@@ -141,7 +141,9 @@ class ElectrodynamicRunner:
     def step(self):
         d, s = self.desc, N.stream_handle()
         src, dst = self._cur, 1 - self._cur
-        if self._pending and not (self.tile and (self.esirkepov or self.direct_fused)):
+        # the fused tile kernels read through a pending permutation; anything else
+        # (or a switch to the strided comparison kernel since the re-bin) applies it first
+        if self._pending and not (self.tile and (self.direct_fused or (self.esirkepov and d.variant != 1))):
             self._materialize()
             src, dst = self._cur, 1 - self._cur
         E, Et = self.E[self.ecur], self.E[1 - self.ecur]

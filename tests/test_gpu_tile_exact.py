@@ -111,6 +111,39 @@ def test_gathered_read_equals_permute_then_push(tile, rounds, min_blocks, dt):
     assert float((Ja - Jb).norm() / Jb.norm()) <= 1e-6
 
 
+def test_pending_permutation_is_applied_before_the_strided_kernel():
+    """A re-bin leaves its permutation pending; switching to the strided
+    comparison kernel (variant 1, which has no gathered read) before the next
+    step applies the permutation first, giving the particles of a runner that
+    read its state in between."""
+    specs = [workloads.SpeciesSpec(12, weight=0.01, vth=(0.15,) * 3)]
+    runs = []
+    for read_between in (False, True):
+        r, *_ = _runner((16, 12, 10), (16.0, 12.0, 10.0), specs, 4, 4, 1)
+        r.step()
+        r.sort()
+        assert r._pending
+        if read_between:
+            r.state()
+            assert not r._pending
+        r.desc.variant = 1
+        r.step()
+        r.desc.variant = 0
+        assert not r._pending
+        parts, _ = r.state()
+        x = parts.x.reshape(-1, 3)
+        u = parts.u.reshape(-1, 3)
+        act = parts.active.reshape(-1)
+        x, u = x[act], u[act]
+        order = torch.argsort(x[:, 0])      # the two re-bins may order equal keys differently
+        runs.append((x[order].clone(), u[order].clone(), int(act.sum())))
+    (xa, ua, na), (xb, ub, nb) = runs
+    # J of the first step sums in atomic order, so E and the second push agree to FP32 rounding
+    assert na == nb
+    assert float((xa - xb).abs().max()) <= 1e-5
+    assert float((ua - ub).abs().max()) <= 1e-5 * float(ua.abs().max())
+
+
 def test_gathered_read_needs_an_exact_grid():
     specs = [workloads.SpeciesSpec(8, weight=0.01, vth=(0.1,) * 3)]
     r, *_ = _runner((12, 10, 8), (1.3, 1.1, 0.7), specs, 4, 4, 1, ext=False)
