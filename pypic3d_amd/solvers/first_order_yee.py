@@ -135,12 +135,17 @@ def yee_e(desc, E3, B3, J3, out3, stream, scratch=None, pml=None):
             N.call(N.op("pic_filter_digital", f64), desc, N.ptr(c), N.ptr(t), float(desc.alpha), stream)
             c.copy_(t)
     walls = {0: (1, 2), 1: (0, 2), 2: (0, 1)}
+    changed = desc.alpha != 1.0
     for axis in range(3):
         if desc.bc[axis] == 1:
+            changed = True
             for comp in walls[axis]:
                 N.call(N.op("pic_zero_boundary", f64), desc, N.ptr(out3[comp]), axis, stream)
-    for c in out3:
-        N.call(N.op("pic_halo_refresh", f64), desc, N.ptr(c), 0, stream)
+    # the update kernels refresh the guard cells themselves; only a filter or the
+    # conducting walls change the interior after that
+    if changed:
+        for c in out3:
+            N.call(N.op("pic_halo_refresh", f64), desc, N.ptr(c), 0, stream)
     return out3
 
 
