@@ -202,7 +202,7 @@ int pic_fused_push_esirkepov_tiled(const PicDesc* desc, const PicSpecies* sp, co
  * Results equal pic_permute_particles(in, gather) followed by
  * pic_fused_push_esirkepov_tiled: the particles bit for bit, J up to FP32
  * summation order.  Exact grids only (pic_tile_exact_grid, desc->variant !=
- * 1); -1 otherwise. */
+ * 1); -1 otherwise.  gather may be NULL for an empty set. */
 int pic_fused_push_esirkepov_tiled_gather(const PicDesc* desc, const PicSpecies* sp, const PicParticles* in,
                                           const int32_t* gather, const float* const E[3], const float* const B[3],
                                           const float* const E_ext[3], const float* const B_ext[3],

@@ -1164,7 +1164,7 @@ int pic_fused_push_direct_tiled_gather(const PicDesc* desc, const PicSpecies* sp
                                        const float* const E_ext[3], const float* const B_ext[3], PicParticles* out,
                                        float* const J[3], int32_t tile, const int32_t* starts, int32_t min_blocks,
                                        int32_t* workspace, void* stream) {
-    PIC_REQUIRE(gather);
+    PIC_REQUIRE(sp && (gather || (int64_t)sp->count * sp->slots == 0));
     return fused_direct_tiled(desc, sp, in, E, B, E_ext, B_ext, out, J, tile, starts, min_blocks, workspace, gather,
                               stream);
 }
@@ -1280,7 +1280,8 @@ int pic_fused_push_esirkepov_tiled_gather(const PicDesc* desc, const PicSpecies*
                                           PicParticles* out, float* const J[3], int32_t tile, const int32_t* starts,
                                           int32_t rounds, int32_t min_blocks, int32_t* workspace, void* stream) {
     ExactGrid eg;
-    PIC_REQUIRE(gather && desc && desc_ok(desc) && desc->variant != 1 && make_exact_grid(*desc, eg));
+    PIC_REQUIRE(desc && desc_ok(desc) && sp && desc->variant != 1 && make_exact_grid(*desc, eg));
+    PIC_REQUIRE(gather || (int64_t)sp->count * sp->slots == 0);
     return fused_esirkepov_tiled(desc, sp, in, E, B, E_ext, B_ext, out, J, tile, starts, rounds, min_blocks,
                                  workspace, gather, stream);
 }

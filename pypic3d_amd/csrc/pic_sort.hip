@@ -509,9 +509,11 @@ int pic_tile_rebin(const PicDesc* desc, const PicSpecies* sp, const PicParticles
                            kc, (float)lookahead, pos, keys, n);
         PIC_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(scan_sums_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, pos, bins, sums);
-    hipLaunchKernelGGL(scan_block_sums_kernel, dim3(1), dim3(kBlock), 0, s, sums, blocks);
-    hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, pos, bins, sums);
+    if (blocks > 0) {       // no species: no bins, only the closing entry of starts
+        hipLaunchKernelGGL(scan_sums_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, pos, bins, sums);
+        hipLaunchKernelGGL(scan_block_sums_kernel, dim3(1), dim3(kBlock), 0, s, sums, blocks);
+        hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, pos, bins, sums);
+    }
     hipLaunchKernelGGL(rebin_starts_kernel, dim3(grid_1d(nb, kBlock)), dim3(kBlock), 0, s, pos, tiles * vol, vol,
                        (int32_t)tiles, (int32_t)sp->count, n, starts);
     PIC_CHECK_LAUNCH();

@@ -151,3 +151,38 @@ def test_pml_absorbs_the_particle_free_1d_wave():
             assert np.isfinite(e)
             assert abs(e - oracle_energy()) <= 1e-4 * e0, step
     assert e < 0.65 * e0
+
+
+@pytest.mark.parametrize("S", [1, 0])
+def test_rebin_and_gathered_launches_on_empty_sets(S):
+    """The re-bin without its gather and the launches that read through its
+    permutation accept an empty set: the ranges are all empty and nothing is
+    read or written."""
+    from pypic3d_amd import _native as N
+    from pypic3d_amd._tensors import make_desc, make_species, particle_planes, pic_particles
+    static, dyn = workloads.make_parameters((8, 8, 8), cfl=0.5)
+    parts, cfg = _empty(S)
+    desc, sp = make_desc(static, dyn), make_species(cfg, 0)
+    assert int(N.lib().pic_tile_exact_grid(desc)) == 1
+    xp, up, ap = particle_planes(parts)
+    pp = pic_particles(xp, up, ap)
+    tiles = int(N.lib().pic_tile_count(desc, 4))
+    starts = torch.full((S * (tiles + 1) + 1,), -1, dtype=torch.int32, device="cuda")
+    ws = torch.empty(max(1, int(N.lib().pic_tile_rebin_workspace_bytes(desc, sp, 4))), dtype=torch.uint8,
+                     device="cuda")
+    s = N.stream_handle()
+    N.call("pic_tile_rebin", desc, sp, pp, 4, 0.0, None, N.ptr(starts), None, N.ptr(ws), ws.numel(), s)
+    torch.cuda.synchronize()
+    assert bool((starts == 0).all())
+    E, B, J, *_ = workloads.empty_fields(static, dyn)
+    f3 = lambda t: [c.view(c.shape[-3:]) for c in t]      # noqa: E731
+    E3, B3, J3 = f3(E), f3(B), f3(J)
+    rare = torch.zeros(1, dtype=torch.int32, device="cuda")
+    for c in J3:
+        c.fill_(0.0)
+    N.call("pic_fused_push_esirkepov_tiled_gather", desc, sp, pp, None, N.p3(E3), N.p3(B3), None, None, pp,
+           N.p3(J3), 4, N.ptr(starts), 4, 4, N.ptr(rare), s)
+    N.call("pic_fused_push_direct_tiled_gather", desc, sp, pp, None, N.p3(E3), N.p3(B3), None, None, pp,
+           N.p3(J3), 4, N.ptr(starts), 4, N.ptr(rare), s)
+    torch.cuda.synchronize()
+    assert all(float(c.abs().sum()) == 0.0 for c in J3)
